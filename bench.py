@@ -1,0 +1,152 @@
+"""Benchmark of the stereo hot path on one MI355X.
+
+Workload (the headline metric's configuration): one 1280x720 uint8 pair from
+the seeded generator (12 rectangles, disparities in [0, 127]), Birchfield-
+Tomasi cost, D = 128, 8-path SGM (P1 = 8, P2 = 32), lowest-index WTA. A step
+is one pass of the path over that pair with the images already in HBM.
+Prints one JSON line.
+
+  python bench.py [--steps K] [--warmup W] [--mode fused|staged]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def b_alg(W, H, D):
+    """Algorithmic bytes per pair of the staged uint16 model: C write + C read
+    + S write + S read per cell (8 B) and L, R reads + disparity write per
+    pixel (4 B)."""
+    return 8 * W * H * D + 4 * W * H
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--mode", choices=["fused", "staged"], default="fused")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every pair and stream is independent, so N GPUs "
+              "would each run their own share; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import thundervision_amd.tdvh as tv
+    from thundervision_amd import synth
+
+    spec = synth.PLAIN_720P
+    W, H, D = spec["width"], spec["height"], spec["depth"]
+    L, R, _ = synth.workload_pair(spec)
+    dev = torch.device("cuda:0")
+    Ld = torch.from_numpy(L).to(dev)
+    Rd = torch.from_numpy(R).to(dev)
+    disp = torch.empty((H, W), dtype=torch.int16, device=dev)
+    ctx = tv.Context(0)
+    stream = torch.cuda.current_stream()
+    fusion = tv.FUSE_COST if args.mode == "fused" else tv.FUSE_STAGED
+
+    def step(fz=fusion):
+        tv.stereo_bt_sgm(Ld, Rd, D, fusion=fz, out=disp, ctx=ctx, stream=stream)
+
+    def timed(fz, steps, warmup):
+        for _ in range(warmup):
+            step(fz)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step(fz)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    cells = W * H * D
+    sec = timed(fusion, args.steps, args.warmup)
+    value = cells / sec / 1e6
+    other = tv.FUSE_STAGED if fusion == tv.FUSE_COST else tv.FUSE_COST
+    sec_other = timed(other, max(5, args.steps // 5), 2)
+
+    # per-kernel hipEvent timing on the launch stream
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(max(5, args.steps // 5)):
+        step()
+    torch.cuda.synchronize()
+    table = ctx.profile_table()
+    ctx.profile(False)
+    dom = max(table, key=lambda k: table[k][0])
+    dom_ms = table[dom][0] / table[dom][1]
+    achieved = b_alg(W, H, D) / (dom_ms * 1e-3) / 1e9
+    pipeline_ms = sum(v[0] / v[1] for v in table.values())
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            want = oracle.stereo_bt_sgm(L, R, D)
+        cs = (time.perf_counter() - t0) / reps
+        cpu = {"value": round(cells / cs / 1e6, 3), "unit": "MDE/s", "cores": 1, "kind": "port",
+               "sample": "%d full 1280x720 D=128 pairs, BT+SGM8+WTA, oracle/tdvref.c gcc -O3 "
+                         "single thread (%s)" % (reps, cpu_model())}
+        got = disp.cpu().numpy()
+        if not (got == want).all():
+            print("ERROR: GPU disparities differ from the oracle", file=sys.stderr)
+            sys.exit(1)
+
+    out = {
+        "metric": "MDE/s (W*H*D cells per second), SGM-BT 8-path 1280x720 D=128",
+        "value": round(value, 1),
+        "unit": "MDE/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u16",
+        "data": "synthetic (seeded generator: blurred noise, 12 rectangles, shift warp, sigma-2 "
+                "noise), bit-exact vs oracle",
+        "config": {"workload": "plain pair 1280x720 uint8, BT2 cost, D=128, SGM 8 paths "
+                               "P1=8 P2=32, lowest-index WTA",
+                   "path": args.mode, "width": W, "height": H, "depth": D},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": dom,
+                     "kernel_avg_ms": round(dom_ms, 4),
+                     "alg_bytes_per_launch": b_alg(W, H, D)},
+        "pipeline_frac_of_hbm_roofline": round(b_alg(W, H, D) / (sec * 1e9) / HBM_PEAK_GBS, 4),
+        "other_path": {"path": "staged" if other == tv.FUSE_STAGED else "fused",
+                       "value": round(cells / sec_other / 1e6, 1),
+                       "ms_per_step": round(sec_other * 1e3, 4)},
+        "kernels_ms": {k: round(v[0] / v[1], 4) for k, v in table.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

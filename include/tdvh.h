@@ -1,0 +1,252 @@
+/*
+ * tdvh.h — C ABI of the MI355X (gfx950) stereo-matching hot path.
+ *
+ * Plain C: pointers, sizes and status codes. Device pointers are HIP device
+ * addresses; `stream` is a hipStream_t passed as void* (NULL = the null
+ * stream). All calls are asynchronous on `stream` unless stated otherwise.
+ * Every call returns TDVH_OK (0) or a negative tdvh_status; the message of
+ * the last failure on the calling thread is available from
+ * tdvh_last_error().
+ *
+ * Each entry point names the reference function it replaces (paths relative
+ * to LibThunderVision/src/tdvision/ of otaviog/ThunderVision).
+ */
+#ifndef TDVH_H
+#define TDVH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status ----------------------------------------------------------- */
+typedef enum {
+    TDVH_OK = 0,
+    TDVH_ERR_INVALID_ARG = -1,
+    TDVH_ERR_UNSUPPORTED = -2,
+    TDVH_ERR_DEVICE = -3,      /* a HIP runtime call failed */
+    TDVH_ERR_OUT_OF_MEMORY = -4,
+    TDVH_ERR_BUSY = -5         /* admission refused: budget exhausted */
+} tdvh_status;
+
+/* Message of the last failed call on this thread ("" if none).
+ * Replaces the CUException text of cuerr.hpp:73-78. */
+const char *tdvh_last_error(void);
+
+/* ---- context ---------------------------------------------------------- */
+/* Per-device context owning scratch memory (the LocalDSIMem / TmpCudaMem
+ * role of dsimem.cpp:22-42 and tmpcudamem.cpp) and cached path tables. A
+ * context is used by one host thread at a time; several contexts may run
+ * concurrently. */
+typedef struct tdvh_ctx tdvh_ctx;
+
+int tdvh_ctx_create(int device, tdvh_ctx **out);
+int tdvh_ctx_destroy(tdvh_ctx *ctx);
+/* Releases cached scratch (SemiGlobalDev::finished, semiglobaldev.cpp:30-35) */
+int tdvh_ctx_trim(tdvh_ctx *ctx);
+/* Bytes of device scratch currently held by the context. */
+size_t tdvh_ctx_scratch_bytes(const tdvh_ctx *ctx);
+
+/* ---- stage timing (CudaBenchmarker, benchmark.cpp:7-31) ---------------- */
+/* While enabled, every kernel this context launches is bracketed by a pair of
+ * hipEvents recorded on the launch stream; durations accumulate per kernel
+ * name. Reading synchronises on the recorded events. */
+int tdvh_profile_enable(tdvh_ctx *ctx, int on);
+int tdvh_profile_reset(tdvh_ctx *ctx);
+/* Entry `index` of the per-kernel table: name, total milliseconds and launch
+ * count. Returns TDVH_ERR_INVALID_ARG past the last entry. */
+int tdvh_profile_read(tdvh_ctx *ctx, int index, char *name, size_t name_len,
+                      double *total_ms, int64_t *launches);
+
+/* ---- disparity-space image (DSIMem, dsimem.hpp:12-136) ---------------- */
+typedef enum {
+    TDVH_U16 = 1, /* BT2 costs, SGM volumes; invalid = 0xFFFF */
+    TDVH_U32 = 2, /* SSD costs; invalid = 0xFFFFFFFF */
+    TDVH_F32 = 3, /* NCC costs 1 - rho; invalid = +inf */
+    TDVH_I32 = 4  /* reference-geometry SGM volume; invalid = 0x3FFFFFFF */
+} tdvh_dtype;
+
+typedef enum {
+    /* d contiguous; element (x, y, d) at x*stride_x + y*stride_y + d.
+     * The reference layout [x][y][d] (dsimem.cpp:34-35, dsimemutil.h:6) is
+     * stride_x = H*D, stride_y = D; row-major is stride_x = D,
+     * stride_y = W*D. */
+    TDVH_LAYOUT_STRIDED = 0,
+    /* Device-native uint16 layout read and written by the SGM kernels:
+     * pixels row-major; each pixel is Dp = 32*ceil(D/32) uint16 stored as
+     * 16*m uint32 words (m = Dp/32); word j*m + r holds d = j*m + r in its
+     * low half and d = Dp/2 + j*m + r in its high half; d >= D is invalid.
+     * stride_x/stride_y are ignored. */
+    TDVH_LAYOUT_SPLIT16 = 1
+} tdvh_layout;
+
+typedef struct {
+    void *ptr;
+    int32_t width, height, depth;
+    int32_t dtype;  /* tdvh_dtype */
+    int32_t layout; /* tdvh_layout */
+    int64_t stride_x, stride_y; /* elements; TDVH_LAYOUT_STRIDED only */
+} tdvh_dsi;
+
+/* Bytes needed by a DSI of this shape/dtype/layout (strides row-major). */
+size_t tdvh_dsi_bytes(int width, int height, int depth, int dtype, int layout);
+/* Fills `out` as a row-major strided or split DSI over `ptr`. */
+int tdvh_dsi_init(tdvh_dsi *out, void *ptr, int width, int height, int depth,
+                  int dtype, int layout);
+/* Copies/reorders a DSI between layouts (same dtype, same shape). */
+int tdvh_dsi_convert(tdvh_ctx *ctx, const tdvh_dsi *src, tdvh_dsi *dst,
+                     void *stream);
+
+/* ---- matching costs (MatchingCost stages) ----------------------------- */
+/* Images are uint8, `pitch` bytes per row, W x H, device memory.
+ * Cells with x - d < 0 are invalid (birchfieldcost.cu:52). Out-of-image
+ * window reads clamp to the edge (texture semantics, birchfieldcost.cu:160). */
+
+/* Birchfield-Tomasi in 2x units (0..510), uint16 output, STRIDED or SPLIT16.
+ * Replaces BirchfieldCostRun (birchfieldcost.cu:194-212). */
+int tdvh_cost_bt(tdvh_ctx *ctx, const uint8_t *left, const uint8_t *right,
+                 int width, int height, int pitch, tdvh_dsi *out, void *stream);
+
+/* (2*radius+1)^2 sum of squared differences, uint32 STRIDED output.
+ * Replaces SSDDevRun (ssd.cu:79-114; radius 3 there). */
+int tdvh_cost_ssd(tdvh_ctx *ctx, const uint8_t *left, const uint8_t *right,
+                  int width, int height, int pitch, int radius, tdvh_dsi *out,
+                  void *stream);
+
+/* Normalised cross-correlation cost 1 - rho in FP32, STRIDED output;
+ * zero_mean = 0 is the reference's plain NCC (crosscorrelation.cu:11-28),
+ * 1 the zero-mean variant. A zero denominator gives +inf (invalid).
+ * Replaces CrossCorrelationDevRun (crosscorrelation.cu:53-76). */
+int tdvh_cost_ncc(tdvh_ctx *ctx, const uint8_t *left, const uint8_t *right,
+                  int width, int height, int pitch, int radius, int zero_mean,
+                  tdvh_dsi *out, void *stream);
+
+/* ---- optimisers (Optimizer stages) ------------------------------------ */
+
+typedef enum {
+    /* reference-geometry quirks (semiglobalcpu.cpp, semiglobal.cu) */
+    TDVH_SGM_REF_GEOMETRY = 1,  /* path table of semiglobal.cu:57-230 */
+    TDVH_SGM_START_ASSIGN = 2,  /* semiglobalcpu.cpp:81 */
+    TDVH_SGM_MIN_SKIP_LAST = 4  /* semiglobalcpu.cpp:95-97 */
+} tdvh_sgm_flags;
+
+typedef struct {
+    int32_t p1, p2;   /* constant penalties */
+    int32_t dir_mask; /* bit i: direction i, order (1,0) (-1,0) (0,1) (0,-1)
+                         (1,1) (-1,-1) (-1,1) (1,-1); 0xFF = all 8 */
+    int32_t flags;    /* tdvh_sgm_flags; 0 = standard geometry */
+} tdvh_sgm_params;
+
+/* Semi-global aggregation + lowest-index WTA. C is uint16 (STRIDED or
+ * SPLIT16). `S_out` (optional, may be NULL) receives the aggregated volume:
+ * uint16 in standard geometry, int32 with TDVH_SGM_REF_GEOMETRY. `disp`
+ * receives int16 disparities (row pitch in elements).
+ * Replaces SemiGlobalDevRun (semiglobal.cu:474-521). */
+int tdvh_sgm(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params *params,
+             tdvh_dsi *S_out, int16_t *disp, int disp_pitch, void *stream);
+
+typedef enum {
+    TDVH_WTA_LOWEST = 0, /* lowest d among equal minima (wtacpu.cpp:28) */
+    TDVH_WTA_TREE = 1    /* pairwise tree, ties to the right operand
+                            (wta.cu:62-78); D must be a power of two */
+} tdvh_wta_mode;
+
+/* Winner-take-all over any dtype/layout. For F32 the minimum cost wins.
+ * Replaces WTARunDev (wta.cu:90-100). */
+int tdvh_wta(tdvh_ctx *ctx, const tdvh_dsi *S, int mode, int16_t *disp,
+             int disp_pitch, void *stream);
+
+typedef enum {
+    TDVH_DP_START_ZERO = 1,    /* dynamicprogcpu.cpp:67-79 */
+    TDVH_DP_REF_BACKTRACK = 2  /* dynamicprogcpu.cpp:81-91 */
+} tdvh_dp_flags;
+
+/* Scanline dynamic programming over uint32 costs (STRIDED) with occlusion
+ * penalty `p_occ` on +-1 transitions and the tie rule of
+ * dynamicprogcpu.cpp:42-56; 64-bit accumulation.
+ * Replaces DynamicProgDevRun (dynamicprog.cu:106-122). */
+int tdvh_dp(tdvh_ctx *ctx, const tdvh_dsi *C, uint32_t p_occ, int flags,
+            int16_t *disp, int disp_pitch, void *stream);
+
+/* ---- fused matchers --------------------------------------------------- */
+
+/* SSD window + lowest-index WTA without a cost volume.
+ * Replaces FastWTADevRun (fastwta.cu:57-83). */
+int tdvh_fast_wta_ssd(tdvh_ctx *ctx, const uint8_t *left,
+                      const uint8_t *right, int width, int height, int pitch,
+                      int depth, int radius, int16_t *disp, int disp_pitch,
+                      void *stream);
+
+typedef enum {
+    TDVH_FUSE_STAGED = 0, /* BT volume in HBM, then SGM, then WTA */
+    TDVH_FUSE_COST = 1    /* BT recomputed inside the SGM scan; no C volume */
+} tdvh_fusion;
+
+/* BT2 + 8-direction SGM + WTA for one pair, from images to disparities.
+ * Replaces SemiGlobalLabDevRun (semiglobal_lab.cu:232-282). */
+int tdvh_stereo_bt_sgm(tdvh_ctx *ctx, const uint8_t *left,
+                       const uint8_t *right, int width, int height, int pitch,
+                       int depth, const tdvh_sgm_params *params, int fusion,
+                       int16_t *disp, int disp_pitch, void *stream);
+
+/* int16 disparity -> float d/D (the value convention of wtacpu.cpp:35-36 and
+ * wta.cu:141 that downstream stages such as RGBConv expect). */
+int tdvh_disp_to_unit_float(tdvh_ctx *ctx, const int16_t *disp, int disp_pitch,
+                            int width, int height, int depth, float *out,
+                            int out_pitch, void *stream);
+
+/* uint8 gray from float [0,1] (round(v*255), clamped): the integer profile's
+ * view of the reference's FloatImage inputs (misc.cpp:18-49). */
+int tdvh_unit_float_to_u8(tdvh_ctx *ctx, const float *src, int src_pitch,
+                          int width, int height, uint8_t *dst, int dst_pitch,
+                          void *stream);
+
+/* ---- reference path table (SGPaths::getDescCPU, semiglobal.cu:57-81) -- */
+typedef struct {
+    uint16_t sx, sy, ex, ey;
+    int16_t dx, dy;
+    uint16_t size;
+    uint16_t pad;
+} tdvh_sgpath;
+
+/* Host function. Returns the path count W + H + 2(W + H - 1); fills `paths`
+ * when not NULL. */
+size_t tdvh_sgm_paths(int width, int height, tdvh_sgpath *paths);
+
+/* ---- batched persistent execution ------------------------------------- */
+/* Many pairs of uneven size and disparity range in one persistent launch
+ * fed by a device work queue of (pair, direction, line-group) tiles.
+ * There is no counterpart in the reference; it runs the same arithmetic as
+ * tdvh_stereo_bt_sgm on every pair. */
+typedef struct {
+    const uint8_t *left, *right; /* device */
+    int32_t width, height, pitch, depth;
+    int16_t *disp;               /* device, row pitch = width */
+} tdvh_pair;
+
+typedef struct tdvh_batch tdvh_batch;
+
+/* Plans a batch (work queue, scratch) for `n` pairs. The pair descriptors
+ * are copied; their device buffers must stay valid while the batch runs. */
+int tdvh_batch_create(tdvh_ctx *ctx, const tdvh_pair *pairs, int n,
+                      const tdvh_sgm_params *params, tdvh_batch **out);
+int tdvh_batch_destroy(tdvh_batch *b);
+/* Tile configuration (autotuner knobs): lines per workgroup, waves per
+ * workgroup, and prefetch depth; 0 keeps the current value. */
+int tdvh_batch_config(tdvh_batch *b, int lines_per_wave, int waves_per_group,
+                      int prefetch);
+/* Enqueues the whole batch on `stream`. With use_graph != 0 the launch
+ * sequence is captured once into a hipGraph and replayed. */
+int tdvh_batch_run(tdvh_batch *b, int use_graph, void *stream);
+/* Total W*H*D cells of the batch. */
+int64_t tdvh_batch_cells(const tdvh_batch *b);
+/* Device bytes held by the batch. */
+size_t tdvh_batch_scratch_bytes(const tdvh_batch *b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TDVH_H */

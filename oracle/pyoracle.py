@@ -1,0 +1,153 @@
+"""numpy wrappers over liboracle.so (tdvref.c). Test infrastructure only."""
+import ctypes as ct
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+INV16 = 0xFFFF
+INV32 = 0xFFFFFFFF
+INF_I32 = 0x3FFFFFFF
+SGM_START_ASSIGN = 1
+SGM_MIN_SKIP_LAST = 2
+DP_START_ZERO = 1
+DP_REF_BACKTRACK = 2
+DIR_DX = (1, -1, 0, 0, 1, -1, -1, 1)
+DIR_DY = (0, 0, 1, -1, 1, -1, 1, -1)
+
+
+def build():
+    subprocess.run(["make", "-s", "-B", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        _LIB = ct.CDLL(path)
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(ct.c_void_p)
+
+
+def _img(a):
+    a = np.ascontiguousarray(a, dtype=np.uint8)
+    assert a.ndim == 2
+    return a
+
+
+def cost_bt2(L, R, D):
+    L, R = _img(L), _img(R)
+    H, W = L.shape
+    C = np.empty((H, W, D), np.uint16)
+    lib().tdvref_cost_bt2(_p(L), _p(R), W, H, D, _p(C))
+    return C
+
+
+def cost_ssd(L, R, D, radius=3, box=True):
+    L, R = _img(L), _img(R)
+    H, W = L.shape
+    C = np.empty((H, W, D), np.uint32)
+    fn = lib().tdvref_cost_ssd_box if box else lib().tdvref_cost_ssd
+    fn(_p(L), _p(R), W, H, D, radius, _p(C))
+    return C
+
+
+def cost_ncc(L, R, D, radius=4, zero_mean=True):
+    """Returns (cost = 1 - rho as float64, rho) with +inf / NaN where invalid."""
+    L, R = _img(L), _img(R)
+    H, W = L.shape
+    C = np.empty((H, W, D), np.float64)
+    rho = np.empty((H, W, D), np.float64)
+    lib().tdvref_cost_ncc(_p(L), _p(R), W, H, D, radius, int(zero_mean), _p(C), _p(rho))
+    return C, rho
+
+
+def sgm_lr(C, P1, P2, direction):
+    C = np.ascontiguousarray(C, np.uint16)
+    H, W, D = C.shape
+    Lr = np.empty_like(C)
+    lib().tdvref_sgm_lr(_p(C), W, H, D, P1, P2, direction, _p(Lr))
+    return Lr
+
+
+def sgm_std(C, P1, P2, dir_mask=0xFF):
+    C = np.ascontiguousarray(C, np.uint16)
+    H, W, D = C.shape
+    S = np.empty_like(C)
+    lib().tdvref_sgm_std(_p(C), W, H, D, P1, P2, dir_mask, _p(S))
+    return S
+
+
+def sgm_ref(C, P1, P2, flags=SGM_START_ASSIGN | SGM_MIN_SKIP_LAST):
+    C = np.ascontiguousarray(C, np.uint16)
+    H, W, D = C.shape
+    S = np.empty((H, W, D), np.int32)
+    lib().tdvref_sgm_ref(_p(C), W, H, D, P1, P2, flags, _p(S))
+    return S
+
+
+def sgm_paths(W, H):
+    n = lib().tdvref_sgm_path_count(W, H)
+    lib().tdvref_sgm_path_count.restype = ct.c_size_t
+    n = lib().tdvref_sgm_path_count(W, H)
+    a = np.zeros((n, 8), np.uint16)
+    lib().tdvref_sgm_paths(W, H, _p(a))
+    out = a.astype(np.int32)
+    out[:, 4:6] = a[:, 4:6].view(np.int16)
+    return out  # columns sx sy ex ey dx dy size pad
+
+
+def sgm_ref_visits(W, H):
+    v = np.empty((8, H, W), np.int32)
+    lib().tdvref_sgm_ref_visits(W, H, _p(v))
+    return v
+
+
+def wta(S):
+    S = np.ascontiguousarray(S)
+    H, W, D = S.shape
+    disp = np.empty((H, W), np.int16)
+    fn = {np.dtype(np.uint16): lib().tdvref_wta_u16, np.dtype(np.uint32): lib().tdvref_wta_u32,
+          np.dtype(np.int32): lib().tdvref_wta_i32, np.dtype(np.float64): lib().tdvref_wta_f64,
+          np.dtype(np.float32): lib().tdvref_wta_f32}[S.dtype]
+    fn(_p(S), W, H, D, _p(disp))
+    return disp
+
+
+def wta_tree(S):
+    S = np.ascontiguousarray(S, np.uint16)
+    H, W, D = S.shape
+    disp = np.empty((H, W), np.int16)
+    lib().tdvref_wta_tree_u16(_p(S), W, H, D, _p(disp))
+    return disp
+
+
+def dp(C, p_occ=20, flags=0, return_acc=False):
+    C = np.ascontiguousarray(C, np.uint32)
+    H, W, D = C.shape
+    disp = np.empty((H, W), np.int16)
+    acc = np.empty((H, D), np.uint64) if return_acc else None
+    lib().tdvref_dp(_p(C), W, H, D, ct.c_uint32(p_occ), flags, _p(disp),
+                    _p(acc) if acc is not None else None)
+    return (disp, acc) if return_acc else disp
+
+
+def fast_wta_ssd(L, R, D, radius=3):
+    L, R = _img(L), _img(R)
+    H, W = L.shape
+    disp = np.empty((H, W), np.int16)
+    lib().tdvref_fast_wta_ssd(_p(L), _p(R), W, H, D, radius, _p(disp))
+    return disp
+
+
+def stereo_bt_sgm(L, R, D, P1=8, P2=32):
+    """BT2 -> 8-direction standard SGM -> lowest-index WTA."""
+    return wta(sgm_std(cost_bt2(L, R, D), P1, P2))

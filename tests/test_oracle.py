@@ -1,0 +1,308 @@
+"""Pins the CPU oracle (oracle/tdvref.c) against closed forms, symmetries,
+brute force and the reference's own path-table arithmetic. CPU only."""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle
+from thundervision_amd import synth
+
+rng = np.random.default_rng(1234)
+
+
+def rand_img(H, W):
+    return rng.integers(0, 256, (H, W), dtype=np.uint8)
+
+
+# ---- Birchfield-Tomasi -------------------------------------------------------
+
+def test_bt_constant_images_are_zero_or_invalid():
+    L = np.full((5, 9), 77, np.uint8)
+    C = oracle.cost_bt2(L, L, 6)
+    for x in range(9):
+        for d in range(6):
+            want = 0 if x - d >= 0 else oracle.INV16
+            assert (C[:, x, d] == want).all()
+
+
+def test_bt_hand_computed_step_edge():
+    # left row 10 10 50 50 50 ; right row = left shifted by one (R(x) = L(x+1))
+    L = np.array([[10, 10, 50, 50, 50]], np.uint8)
+    R = np.array([[10, 50, 50, 50, 50]], np.uint8)
+    C = oracle.cost_bt2(L, R, 2)
+    # x=1, d=0: left pixel 10 (neighbours 10, 50 -> 2x terms 20, 20, 60) range
+    # [20, 60]; right pixel R(1)=50 (neighbours 10, 50 -> 100, 60, 100) range
+    # [60, 100]. l2=20 vs [60,100] -> 40; r2=100 vs [20,60] -> 40 -> 40.
+    assert C[0, 1, 0] == 40
+    # x=2, d=1: left 50 (neighbours 10, 50 -> 100, 60, 100), right R(1)=50
+    # (100, 60, 100): identical -> 0
+    assert C[0, 2, 1] == 0
+    # x=1, d=1: left 10 -> l2=20 range [20,60]; right R(0)=10 (clamped
+    # neighbour 10 and 50 -> 20, 20, 60) -> 0
+    assert C[0, 1, 1] == 0
+    assert C[0, 0, 1] == oracle.INV16
+
+
+def test_bt_mirror_symmetry():
+    # C_{L,R}(x, d) == C_{flip R, flip L}(W-1-x+d, d): BT is symmetric in its
+    # two images and the clamp is mirror-symmetric.
+    H, W, D = 6, 23, 7
+    L, R = rand_img(H, W), rand_img(H, W)
+    C = oracle.cost_bt2(L, R, D)
+    Cm = oracle.cost_bt2(R[:, ::-1], L[:, ::-1], D)
+    for x in range(W):
+        for d in range(D):
+            if x - d < 0:
+                continue
+            assert (C[:, x, d] == Cm[:, W - 1 - x + d, d]).all()
+
+
+def test_bt_range_and_invalid_mask():
+    H, W, D = 7, 31, 12
+    C = oracle.cost_bt2(rand_img(H, W), rand_img(H, W), D)
+    x = np.arange(W)[None, :, None]
+    d = np.arange(D)[None, None, :]
+    inv = np.broadcast_to(x - d < 0, C.shape)
+    assert (C[inv] == oracle.INV16).all()
+    assert (C[~inv] <= 510).all()
+
+
+# ---- SSD / NCC -----------------------------------------------------------------
+
+def test_ssd_box_equals_brute_force():
+    H, W, D = 9, 17, 6
+    L, R = rand_img(H, W), rand_img(H, W)
+    for r in (1, 3):
+        assert (oracle.cost_ssd(L, R, D, r, box=True) == oracle.cost_ssd(L, R, D, r, box=False)).all()
+
+
+def test_ssd_constant_offset_closed_form():
+    L = rand_img(12, 20) // 2
+    R = L + 7
+    C = oracle.cost_ssd(L, R, 1, 3)
+    assert (C[:, :, 0] == 49 * 49).all()
+
+
+def _ncc_brute(L, R, D, r, zero_mean):
+    H, W = L.shape
+    out = np.full((H, W, D), np.inf)
+    for y in range(H):
+        for x in range(W):
+            ys = np.clip(np.arange(y - r, y + r + 1), 0, H - 1)
+            xr = np.arange(x - r, x + r + 1)
+            a = L[np.ix_(ys, np.clip(xr, 0, W - 1))].astype(np.float64)
+            for d in range(D):
+                if x - d < 0:
+                    continue
+                b = R[np.ix_(ys, np.clip(xr - d, 0, W - 1))].astype(np.float64)
+                if zero_mean:
+                    a0, b0 = a - a.mean(), b - b.mean()
+                else:
+                    a0, b0 = a, b
+                den = np.sqrt((a0 * a0).sum() * (b0 * b0).sum())
+                if den > 0:
+                    out[y, x, d] = 1.0 - (a0 * b0).sum() / den
+    return out
+
+
+@pytest.mark.parametrize("zero_mean,r", [(True, 4), (False, 1), (True, 1)])
+def test_ncc_matches_brute_force(zero_mean, r):
+    H, W, D = 11, 14, 5
+    L, R = rand_img(H, W), rand_img(H, W)
+    C, _ = oracle.cost_ncc(L, R, D, r, zero_mean)
+    B = _ncc_brute(L, R, D, r, zero_mean)
+    fin = np.isfinite(B)
+    assert (np.isfinite(C) == fin).all()
+    assert np.allclose(C[fin], B[fin], rtol=0, atol=1e-12)
+
+
+def test_ncc_affine_gives_rho_one_and_flat_is_invalid():
+    L = rand_img(10, 12) // 3 + 1
+    R = (2 * L.astype(np.int32) + 5).astype(np.uint8)
+    C, rho = oracle.cost_ncc(L, R, 1, 2, True)
+    assert np.allclose(rho[:, :, 0], 1.0, atol=1e-12)
+    flat = np.full((6, 6), 9, np.uint8)
+    C, rho = oracle.cost_ncc(flat, flat, 2, 1, True)
+    assert np.isinf(C).all()
+
+
+# ---- SGM -----------------------------------------------------------------------
+
+def test_sgm_zero_penalties_give_eight_times_cost():
+    C = oracle.cost_bt2(rand_img(9, 13), rand_img(9, 13), 5)
+    S = oracle.sgm_std(C, 0, 0)
+    v = C != oracle.INV16
+    assert (S[v] == 8 * C[v]).all() and (S[~v] == oracle.INV16).all()
+
+
+def test_sgm_zero_volume():
+    C = np.zeros((6, 7, 4), np.uint16)
+    assert (oracle.sgm_std(C, 8, 32) == 0).all()
+
+
+def _lr_bruteforce(c, P1, P2):
+    """Path costs along one scan line by exhaustive enumeration of disparity
+    sequences: U(k, d) = min over sequences ending in d of sum C + penalties
+    (0 same, P1 for +-1, P2 otherwise); L(k, d) = U(k, d) - min U(k-1, .)."""
+    n, D = c.shape
+    INF = float("inf")
+    U = np.full((n, D), INF)
+    for k in range(n):
+        for seq in itertools.product(range(D), repeat=k + 1):
+            tot = 0.0
+            for i, d in enumerate(seq):
+                v = c[i, d]
+                if v == oracle.INV16:
+                    tot = INF
+                    break
+                tot += v
+                if i:
+                    jump = abs(d - seq[i - 1])
+                    tot += 0 if jump == 0 else (P1 if jump == 1 else P2)
+            U[k, seq[-1]] = min(U[k, seq[-1]], tot)
+    L = np.empty_like(U)
+    L[0] = U[0]
+    for k in range(1, n):
+        L[k] = U[k] - U[k - 1][np.isfinite(U[k - 1])].min()
+    return L
+
+
+def test_sgm_single_direction_matches_path_enumeration():
+    H, W, D = 3, 5, 4
+    C = oracle.cost_bt2(rand_img(H, W), rand_img(H, W), D)
+    C[C != oracle.INV16] = rng.integers(0, 60, (C != oracle.INV16).sum())
+    Lr = oracle.sgm_lr(C, 5, 17, 0)  # direction (1,0)
+    for y in range(H):
+        want = _lr_bruteforce(C[y], 5, 17)
+        got = Lr[y].astype(np.float64)
+        got[Lr[y] == oracle.INV16] = np.inf
+        assert np.array_equal(got, want)
+
+
+def test_sgm_direction_symmetries():
+    H, W, D = 7, 9, 6
+    C = oracle.cost_bt2(rand_img(H, W), rand_img(H, W), D)
+    Cs = rng.integers(0, 300, C.shape).astype(np.uint16)  # no invalid cells
+    flip = lambda v: v[:, ::-1]
+    tr = lambda v: v.transpose(1, 0, 2)
+    # mirror in x maps (1,0)->(-1,0), (1,1)->(-1,1), (1,-1)->(-1,-1)
+    for a, b in ((0, 1), (4, 6), (7, 5), (2, 2)):
+        assert (oracle.sgm_lr(np.ascontiguousarray(flip(Cs)), 8, 32, a) == flip(oracle.sgm_lr(Cs, 8, 32, b))).all()
+    # transpose maps (1,0)->(0,1), (-1,0)->(0,-1), (1,1)->(1,1), (-1,1)->(1,-1)
+    for a, b in ((2, 0), (3, 1), (4, 4), (7, 6)):
+        assert (oracle.sgm_lr(np.ascontiguousarray(tr(Cs)), 8, 32, a) == tr(oracle.sgm_lr(Cs, 8, 32, b))).all()
+
+
+def test_sgm_bounds():
+    C = oracle.cost_bt2(rand_img(20, 30), rand_img(20, 30), 9)
+    for d in range(8):
+        Lr = oracle.sgm_lr(C, 8, 32, d)
+        v = C != oracle.INV16
+        assert (Lr[v] >= C[v]).all() and (Lr[v] <= C[v].astype(int) + 32).all()
+        assert (Lr[~v] == oracle.INV16).all()
+
+
+def test_path_table_counts():
+    for (W, H), n in (((384, 288), 2014), ((1280, 720), 5998), ((1920, 1080), 8998)):
+        assert len(oracle.sgm_paths(W, H)) == n
+
+
+@pytest.mark.parametrize("W,H,dirs,skipped,revisited,maxv", [
+    (384, 288, (5, 7), 27648, 288, 97),
+    (64, 48, (5, 7), 768, 48, 17),
+    (48, 64, (5,), 720, 48, 16),
+])
+def test_reference_path_coverage(W, H, dirs, skipped, revisited, maxv):
+    v = oracle.sgm_ref_visits(W, H)
+    for di in range(8):
+        if di in dirs:
+            assert (v[di] == 0).sum() == skipped
+            assert (v[di] > 1).sum() == revisited
+            assert v[di].max() == maxv
+        else:
+            assert (v[di] == 1).all(), di
+
+
+def test_reference_path_coverage_square_is_exact():
+    assert (oracle.sgm_ref_visits(32, 32) == 1).all()
+
+
+def test_ref_geometry_on_square_equals_standard_without_quirks():
+    C = oracle.cost_bt2(rand_img(12, 12), rand_img(12, 12), 6)
+    S = oracle.sgm_ref(C, 8, 32, flags=0)
+    Sstd = oracle.sgm_std(C, 8, 32)
+    v = C != oracle.INV16
+    assert (S[v] == Sstd[v].astype(np.int32)).all() and (S[~v] == oracle.INF_I32).all()
+
+
+def test_ref_min_skip_last_is_inert_when_last_disparity_invalid():
+    C = oracle.cost_bt2(rand_img(10, 14), rand_img(10, 14), 5)
+    C[:, :, 4] = oracle.INV16
+    a = oracle.sgm_ref(C, 8, 32, flags=oracle.SGM_START_ASSIGN)
+    b = oracle.sgm_ref(C, 8, 32, flags=oracle.SGM_START_ASSIGN | oracle.SGM_MIN_SKIP_LAST)
+    assert (a == b).all()
+
+
+# ---- WTA -----------------------------------------------------------------------
+
+def test_wta_ties_go_to_lowest_index_and_invalid_never_wins():
+    S = np.array([[[5, 3, 3, 9], [7, 7, 7, 7], [0xFFFF, 4, 2, 2], [0xFFFF] * 4]], np.uint16)
+    assert oracle.wta(S).tolist() == [[1, 0, 2, 0]]
+
+
+def test_wta_tree_ties_go_right():
+    # pairs (0,2),(1,3) then (w02, w13); equal values take the right operand
+    S = np.array([[[4, 4, 4, 4], [1, 5, 1, 5], [3, 2, 9, 2]]], np.uint16)
+    # all equal: (0,2)->2, (1,3)->3, (2,3)->3
+    # [1,5,1,5]: (0,2)->2 (tie), (1,3)->3, (1 vs 5) -> left=2
+    # [3,2,9,2]: (0,2)->0, (1,3)->3 (tie), (3 vs 2) -> right=3
+    assert oracle.wta_tree(S).tolist() == [[3, 2, 3]]
+
+
+# ---- DP ------------------------------------------------------------------------
+
+def test_dp_optimal_cost_matches_enumeration():
+    W, D, P = 6, 4, 20
+    for trial in range(5):
+        C = rng.integers(0, 100, (1, W, D)).astype(np.uint32)
+        C[0, 0, 1:] = oracle.INV32
+        disp, acc = oracle.dp(C, P, return_acc=True)
+        best = None
+        for seq in itertools.product(range(D), repeat=W):
+            if any(abs(seq[i] - seq[i - 1]) > 1 for i in range(1, W)):
+                continue
+            if any(C[0, i, d] == oracle.INV32 for i, d in enumerate(seq)):
+                continue
+            tot = sum(int(C[0, i, d]) for i, d in enumerate(seq))
+            tot += P * sum(seq[i] != seq[i - 1] for i in range(1, W))
+            best = tot if best is None else min(best, tot)
+        assert int(acc[0].min()) == best
+        d = disp[0]
+        assert all(abs(int(d[i]) - int(d[i - 1])) <= 1 for i in range(1, W))
+        got = sum(int(C[0, i, d[i]]) for i in range(W)) + P * sum(d[i] != d[i - 1] for i in range(1, W))
+        assert got == best
+
+
+def test_dp_reference_backtrack_is_shifted_by_one():
+    C = rng.integers(0, 1000, (3, 9, 5)).astype(np.uint32)
+    a = oracle.dp(C, 20)
+    b = oracle.dp(C, 20, flags=oracle.DP_REF_BACKTRACK)
+    # the reference writes d(x-1) at x (and d(0) at x = 0)
+    assert (b[:, 1:] == a[:, :-1]).all() and (b[:, 0] == a[:, 0]).all()
+
+
+# ---- generator -----------------------------------------------------------------
+
+def test_generator_known_answers():
+    L, R, gt = synth.workload_pair(synth.PLAIN_384)
+    assert synth.hash_bytes(L) == 0x67DA62008B67E205
+    assert synth.hash_bytes(R) == 0xF098B1E556AD000E
+    assert gt.min() >= 0 and gt.max() <= 15
+    L2, R2, gt2 = synth.workload_pair(synth.PLAIN_384)
+    assert (L == L2).all() and (R == R2).all() and (gt == gt2).all()
+
+
+def test_fast_wta_equals_ssd_then_wta():
+    L, R = rand_img(10, 16), rand_img(10, 16)
+    assert (oracle.fast_wta_ssd(L, R, 5) == oracle.wta(oracle.cost_ssd(L, R, 5))).all()

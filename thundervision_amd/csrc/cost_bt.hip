@@ -1,0 +1,193 @@
+// Birchfield-Tomasi matching cost (BT2, uint16) and DSI layout conversion.
+//
+// BT2 restates birchfieldcost.cu:14-59 in 2x units: for the left pixel x and
+// the right pixel x - d, with v2 = 2I(x), and min/max over {2I(x),
+// I(x)+I(x-1), I(x)+I(x+1)} (edges clamped),
+//   C = min(max(0, l2 - rmax, rmin - l2), max(0, r2 - lmax, lmin - r2)),
+// invalid (0xFFFF) when x - d < 0 (birchfieldcost.cu:52).
+//
+// Data path: one tiny pass packs each pixel's three BT terms into a uint32
+// (2 x W*H words, L2-resident); the cost pass then needs one term word of the
+// left pixel (shared by the 16*m threads of that pixel) and two gathered term
+// words of the right image per output word, and writes the volume with fully
+// coalesced 32-bit stores in the SPLIT16 layout the SGM kernels read.
+#include "kernels.hpp"
+
+namespace tdvh {
+
+__global__ __launch_bounds__(256) void bt_terms_kernel(const uint8_t *__restrict__ L,
+                                                       const uint8_t *__restrict__ R, int W, int H,
+                                                       int pitch, uint32_t *__restrict__ TL,
+                                                       uint32_t *__restrict__ TR)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= W) return;
+    const uint8_t *lr = L + (size_t)y * pitch;
+    const uint8_t *rr = R + (size_t)y * pitch;
+    const int xm = x > 0 ? x - 1 : 0, xp = x < W - 1 ? x + 1 : W - 1;
+    TL[(size_t)y * W + x] = bt_pack(lr[x], lr[xm], lr[xp]);
+    TR[(size_t)y * W + x] = bt_pack(rr[x], rr[xm], rr[xp]);
+}
+
+int launch_bt_terms(const uint8_t *L, const uint8_t *R, int W, int H, int pitch, uint32_t *TL,
+                    uint32_t *TR, hipStream_t s)
+{
+    dim3 grid((W + 255) / 256, H);
+    prof_begin(s);
+    bt_terms_kernel<<<grid, 256, 0, s>>>(L, R, W, H, pitch, TL, TR);
+    return check_launch("bt_terms_kernel", s);
+}
+
+// One thread per output word of the SPLIT16 layout (two cells: d_lo, d_hi).
+template <int M>
+__global__ __launch_bounds__(256) void bt_split_kernel(const uint32_t *__restrict__ TL,
+                                                       const uint32_t *__restrict__ TR, int W,
+                                                       int D, uint32_t *__restrict__ out)
+{
+    constexpr int WPP = 16 * M; // words per pixel
+    const int y = blockIdx.y;
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= W * WPP) return;
+    const int x = w / WPP, jr = w - x * WPP;
+    const int dlo = jr, dhi = 16 * M + jr; // j*M + r == jr
+    const size_t p = (size_t)y * W + x;
+    const uint32_t lt = TL[p];
+    uint32_t clo = INV16, chi = INV16;
+    if (dlo < D && dlo <= x) clo = bt_cost(lt, TR[p - dlo]);
+    if (dhi < D && dhi <= x) chi = bt_cost(lt, TR[p - dhi]);
+    out[p * WPP + jr] = clo | (chi << 16);
+}
+
+// One thread per cell of a strided uint16 volume.
+__global__ __launch_bounds__(256) void bt_strided_kernel(const uint32_t *__restrict__ TL,
+                                                         const uint32_t *__restrict__ TR, int W,
+                                                         int H, int D, int64_t sx, int64_t sy,
+                                                         uint16_t *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)W * H * D;
+    if (i >= n) return;
+    const int d = (int)(i % D);
+    const int64_t px = i / D;
+    const int x = (int)(px % W), y = (int)(px / W);
+    const size_t p = (size_t)y * W + x;
+    const uint32_t c = d <= x ? bt_cost(TL[p], TR[p - d]) : INV16;
+    out[x * sx + y * sy + d] = (uint16_t)c;
+}
+
+int launch_bt_split(const uint32_t *TL, const uint32_t *TR, int W, int H, int D, uint32_t *out,
+                    hipStream_t s)
+{
+    const int M = split_m(D);
+    dim3 grid((W * 16 * M + 255) / 256, H);
+    prof_begin(s);
+    switch (M) {
+#define CASE(m)                                                                                    \
+    case m: bt_split_kernel<m><<<grid, 256, 0, s>>>(TL, TR, W, D, out); break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+    default: set_error("depth %d above %d", D, MAX_DEPTH); return TDVH_ERR_UNSUPPORTED;
+    }
+    return check_launch("bt_split_kernel", s);
+}
+
+// ---- DSI layout conversion (uint16 strided <-> SPLIT16) --------------------
+
+template <bool TO_SPLIT>
+__global__ __launch_bounds__(256) void convert_kernel(uint16_t *strided, uint32_t *split, int W,
+                                                      int H, int D, int M, int64_t sx,
+                                                      int64_t sy)
+{
+    const int WPP = 16 * M;
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= (int64_t)W * H * WPP) return;
+    const int64_t p = w / WPP;
+    const int jr = (int)(w - p * WPP);
+    const int x = (int)(p % W), y = (int)(p / W);
+    const int dlo = jr, dhi = 16 * M + jr;
+    uint16_t *base = strided + x * sx + y * sy;
+    if (TO_SPLIT) {
+        const uint32_t lo = dlo < D ? base[dlo] : INV16;
+        const uint32_t hi = dhi < D ? base[dhi] : INV16;
+        split[w] = lo | (hi << 16);
+    } else {
+        const uint32_t v = split[w];
+        if (dlo < D) base[dlo] = (uint16_t)(v & 0xFFFF);
+        if (dhi < D) base[dhi] = (uint16_t)(v >> 16);
+    }
+}
+
+int launch_convert(uint16_t *strided, int64_t sx, int64_t sy, uint32_t *split, int W, int H,
+                   int D, bool to_split, hipStream_t s)
+{
+    const int M = split_m(D);
+    const int64_t n = (int64_t)W * H * 16 * M;
+    const int blocks = (int)((n + 255) / 256);
+    prof_begin(s);
+    if (to_split)
+        convert_kernel<true><<<blocks, 256, 0, s>>>(strided, split, W, H, D, M, sx, sy);
+    else
+        convert_kernel<false><<<blocks, 256, 0, s>>>(strided, split, W, H, D, M, sx, sy);
+    return check_launch("convert_kernel", s);
+}
+
+} // namespace tdvh
+
+using namespace tdvh;
+
+extern "C" int tdvh_cost_bt(tdvh_ctx *ctx, const uint8_t *left, const uint8_t *right, int width,
+                            int height, int pitch, tdvh_dsi *out, void *stream)
+{
+    clear_error();
+    TDVH_TRY(use_device(ctx));
+    TDVH_REQUIRE(left && right && out && out->ptr, "null pointer argument");
+    TDVH_REQUIRE(width > 0 && height > 0 && pitch >= width, "bad image %dx%d pitch %d", width,
+                 height, pitch);
+    TDVH_REQUIRE(out->width == width && out->height == height && out->depth > 0,
+                 "DSI shape %dx%dx%d does not match image %dx%d", out->width, out->height,
+                 out->depth, width, height);
+    TDVH_REQUIRE(out->dtype == TDVH_U16, "BT output must be uint16");
+    hipStream_t s = (hipStream_t)stream;
+    void *tl = nullptr;
+    TDVH_TRY(ctx->scratch("bt_terms", (size_t)width * height * 8, &tl));
+    uint32_t *TL = (uint32_t *)tl, *TR = TL + (size_t)width * height;
+    TDVH_TRY(launch_bt_terms(left, right, width, height, pitch, TL, TR, s));
+    if (out->layout == TDVH_LAYOUT_SPLIT16) {
+        TDVH_REQUIRE(out->depth <= MAX_DEPTH, "depth %d above %d", out->depth, MAX_DEPTH);
+        return launch_bt_split(TL, TR, width, height, out->depth, (uint32_t *)out->ptr, s);
+    }
+    TDVH_REQUIRE(out->layout == TDVH_LAYOUT_STRIDED, "unknown layout %d", out->layout);
+    const int64_t n = (int64_t)width * height * out->depth;
+    prof_begin(s);
+    bt_strided_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(
+        TL, TR, width, height, out->depth, out->stride_x, out->stride_y, (uint16_t *)out->ptr);
+    return check_launch("bt_strided_kernel", s);
+}
+
+extern "C" int tdvh_dsi_convert(tdvh_ctx *ctx, const tdvh_dsi *src, tdvh_dsi *dst, void *stream)
+{
+    clear_error();
+    TDVH_TRY(use_device(ctx));
+    TDVH_REQUIRE(src && dst && src->ptr && dst->ptr, "null pointer argument");
+    TDVH_REQUIRE(src->width == dst->width && src->height == dst->height &&
+                     src->depth == dst->depth && src->dtype == dst->dtype,
+                 "shape/dtype mismatch");
+    hipStream_t s = (hipStream_t)stream;
+    if (src->layout == dst->layout) {
+        TDVH_REQUIRE(src->layout == TDVH_LAYOUT_SPLIT16 ||
+                         (src->stride_x == dst->stride_x && src->stride_y == dst->stride_y),
+                     "strided-to-strided reorder is not supported");
+        const size_t bytes = tdvh_dsi_bytes(src->width, src->height, src->depth, src->dtype,
+                                            src->layout);
+        TDVH_CHECK_HIP(hipMemcpyAsync(dst->ptr, src->ptr, bytes, hipMemcpyDeviceToDevice, s));
+        return TDVH_OK;
+    }
+    TDVH_REQUIRE(src->dtype == TDVH_U16, "SPLIT16 conversion needs uint16");
+    TDVH_REQUIRE(src->depth <= MAX_DEPTH, "depth %d above %d", src->depth, MAX_DEPTH);
+    if (dst->layout == TDVH_LAYOUT_SPLIT16)
+        return launch_convert((uint16_t *)src->ptr, src->stride_x, src->stride_y,
+                              (uint32_t *)dst->ptr, src->width, src->height, src->depth, true, s);
+    return launch_convert((uint16_t *)dst->ptr, dst->stride_x, dst->stride_y,
+                          (uint32_t *)src->ptr, src->width, src->height, src->depth, false, s);
+}
