@@ -83,7 +83,7 @@ def test_sgm_direction_masks(tv, dev, mask):
     assert (u16(S) == S_want).all()
 
 
-@pytest.mark.parametrize("p1,p2", [(0, 0), (8, 32), (3, 100), (30, 150)])
+@pytest.mark.parametrize("p1,p2", [(0, 0), (8, 32), (3, 100), (30, 150), (8, 255), (8, 300), (40, 1000)])
 def test_sgm_penalties(tv, dev, p1, p2):
     W, H, D = 60, 31, 40
     L, R = pair(W, H, 39, 11)
@@ -177,3 +177,25 @@ def test_errors_are_reported(tv, dev):
     L = torch.zeros((8, 8), dtype=torch.uint8, device=dev)
     with pytest.raises(tv.TdvhError, match="depth"):
         tv.stereo_bt_sgm(L, L, 300)
+
+
+@pytest.mark.parametrize("fusion", [0, 1])
+@pytest.mark.parametrize("W,H,D,p2", [(97, 40, 80, 32), (65, 33, 33, 300), (300, 20, 256, 32),
+                                      (40, 70, 160, 7), (31, 17, 1, 32)])
+def test_stereo_shapes_and_paths(tv, dev, fusion, W, H, D, p2):
+    L, R = pair(W, H, max(D - 1, 0), 3 * W + 7 * H)
+    want = oracle.wta(oracle.sgm_std(oracle.cost_bt2(L, R, D), 8, p2))
+    got = tv.stereo_bt_sgm(to_dev(L, dev), to_dev(R, dev), D, p2=p2, fusion=fusion)
+    assert (got.cpu().numpy() == want).all()
+
+
+def test_stereo_pitched_images(tv, dev):
+    W, H, D = 77, 41, 48
+    L, R = pair(W, H, D - 1, 99)
+    big = torch.zeros((2, H, W + 13), dtype=torch.uint8, device=dev)
+    big[0, :, :W] = to_dev(L, dev)
+    big[1, :, :W] = to_dev(R, dev)
+    want = oracle.stereo_bt_sgm(L, R, D)
+    for fusion in (0, 1):
+        got = tv.stereo_bt_sgm(big[0, :, :W], big[1, :, :W], D, fusion=fusion)
+        assert (got.cpu().numpy() == want).all()

@@ -39,7 +39,8 @@ int launch_bt_terms(const uint8_t *L, const uint8_t *R, int W, int H, int pitch,
     return check_launch("bt_terms_kernel", s);
 }
 
-// One thread per output word of the SPLIT16 layout (two cells: d_lo, d_hi).
+// Output words of the SPLIT16 layout (two cells each: d_lo, d_hi); a block
+// strides over one slice of a row (few large blocks keep dispatch cheap).
 template <int M>
 __global__ __launch_bounds__(256) void bt_split_kernel(const uint32_t *__restrict__ TL,
                                                        const uint32_t *__restrict__ TR, int W,
@@ -47,16 +48,17 @@ __global__ __launch_bounds__(256) void bt_split_kernel(const uint32_t *__restric
 {
     constexpr int WPP = 16 * M; // words per pixel
     const int y = blockIdx.y;
-    const int w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= W * WPP) return;
-    const int x = w / WPP, jr = w - x * WPP;
-    const int dlo = jr, dhi = 16 * M + jr; // j*M + r == jr
-    const size_t p = (size_t)y * W + x;
-    const uint32_t lt = TL[p];
-    uint32_t clo = INV16, chi = INV16;
-    if (dlo < D && dlo <= x) clo = bt_cost(lt, TR[p - dlo]);
-    if (dhi < D && dhi <= x) chi = bt_cost(lt, TR[p - dhi]);
-    out[p * WPP + jr] = clo | (chi << 16);
+    const int n = W * WPP;
+    for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < n; w += gridDim.x * blockDim.x) {
+        const int x = w / WPP, jr = w - x * WPP;
+        const int dlo = jr, dhi = 16 * M + jr; // j*M + r == jr
+        const size_t p = (size_t)y * W + x;
+        const uint32_t lt = TL[p];
+        uint32_t clo = INV16, chi = INV16;
+        if (dlo < D && dlo <= x) clo = bt_cost(lt, TR[p - dlo]);
+        if (dhi < D && dhi <= x) chi = bt_cost(lt, TR[p - dhi]);
+        out[p * WPP + jr] = clo | (chi << 16);
+    }
 }
 
 // One thread per cell of a strided uint16 volume.
@@ -80,7 +82,8 @@ int launch_bt_split(const uint32_t *TL, const uint32_t *TR, int W, int H, int D,
                     hipStream_t s)
 {
     const int M = split_m(D);
-    dim3 grid((W * 16 * M + 255) / 256, H);
+    const int per_row = (W * 16 * M + 255) / 256;
+    dim3 grid(per_row < 8 ? per_row : 8, H);
     prof_begin(s);
     switch (M) {
 #define CASE(m)                                                                                    \

@@ -12,15 +12,10 @@ int launch_bt_split(const uint32_t *TL, const uint32_t *TR, int W, int H, int D,
 int launch_convert(uint16_t *strided, int64_t sx, int64_t sy, uint32_t *split, int W, int H,
                    int D, bool to_split, hipStream_t s);
 
-// Standard-geometry SGM from a SPLIT16 volume (C) or from BT terms (TL/TR
-// non-null, C null) into one SPLIT16 path-cost plane per enabled direction.
-int launch_sgm_lines(const uint32_t *C, const uint32_t *TL, const uint32_t *TR, int W, int H,
-                     int D, int P1, int P2, int dir_mask, uint32_t *Lout, size_t plane_words,
-                     hipStream_t s);
-// Sums `nplanes` path-cost planes, optionally stores S (SPLIT16) and writes
-// the lowest-index argmin as int16 disparities.
-int launch_sgm_sum_wta(const uint32_t *L, size_t plane_words, int nplanes, int W, int H, int D,
-                       uint32_t *S_split, int16_t *disp, int disp_pitch, hipStream_t s);
+// BT term planes of the fused SGM path (splatted left terms, paired right
+// terms; see sgm.hip).
+int launch_bt_planes(const uint8_t *L, const uint8_t *R, int W, int H, int pitch, int D,
+                     uint32_t *LT, uint32_t *RT, hipStream_t s);
 
 // Reference-geometry SGM into an int32 strided [y][x][d] volume.
 int run_sgm_ref(tdvh_ctx *ctx, const uint32_t *C_split, int W, int H, int D, int P1, int P2,
