@@ -1,0 +1,107 @@
+"""GPU parity of the window costs (SSD, NCC), fused SSD+WTA and scanline DP
+against the CPU oracle. SSD/DP/WTA bit-exact; NCC within 1e-5 relative of
+the FP64 oracle."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from thundervision_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+# NCC tolerance: |cost_gpu - cost_fp64| <= 1e-5 * max(|rho|, 1e-2)
+NCC_REL, NCC_FLOOR = 1e-5, 1e-2
+NEAR_TIE = 1e-6
+
+
+def to_dev(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def pair(W, H, dmax, seed):
+    L, R, _ = synth.pair(W, H, dmax, 8, seed)
+    return L, R
+
+
+@pytest.mark.parametrize("W,H,D,r", [(33, 21, 16, 3), (70, 45, 64, 3), (100, 37, 130, 3),
+                                     (64, 64, 256, 3), (9, 40, 5, 1), (50, 33, 70, 5),
+                                     (3, 2, 4, 2)])
+def test_ssd_volume(tv, dev, W, H, D, r):
+    L, R = pair(W, H, D - 1, W + H + r)
+    want = oracle.cost_ssd(L, R, D, r)
+    got = tv.cost_ssd(to_dev(L, dev), to_dev(R, dev), D, r)
+    assert (got.cpu().numpy().view(np.uint32) == want).all()
+
+
+@pytest.mark.parametrize("W,H,D,r", [(61, 29, 32, 3), (140, 50, 200, 3), (17, 80, 64, 2)])
+def test_fast_wta_ssd(tv, dev, W, H, D, r):
+    L, R = pair(W, H, D - 1, 7 * W + H)
+    want = oracle.fast_wta_ssd(L, R, D, r)
+    got = tv.fast_wta_ssd(to_dev(L, dev), to_dev(R, dev), D, r)
+    assert (got.cpu().numpy() == want).all()
+
+
+def _check_ncc(got, C64, rho):
+    fin = np.isfinite(C64)
+    assert (np.isfinite(got) == fin).all()
+    err = np.abs(got[fin].astype(np.float64) - C64[fin])
+    bound = NCC_REL * np.maximum(np.abs(rho[fin]), NCC_FLOOR)
+    assert (err <= bound).all(), float((err / bound).max())
+
+
+@pytest.mark.parametrize("W,H,D,r,zm", [(45, 30, 16, 4, True), (80, 41, 64, 4, True),
+                                        (64, 40, 100, 1, False), (33, 33, 256, 4, True),
+                                        (50, 20, 32, 2, False)])
+def test_ncc_volume(tv, dev, W, H, D, r, zm):
+    L, R = pair(W, H, D - 1, 3 * W + r)
+    C64, rho = oracle.cost_ncc(L, R, D, r, zm)
+    got = tv.cost_ncc(to_dev(L, dev), to_dev(R, dev), D, r, zm).cpu().numpy()
+    _check_ncc(got, C64, rho)
+
+
+def test_ncc_flat_windows_are_invalid(tv, dev):
+    L = np.full((20, 30), 100, np.uint8)
+    R = L.copy()
+    got = tv.cost_ncc(to_dev(L, dev), to_dev(R, dev), 8, 4, True).cpu().numpy()
+    assert np.isinf(got).all()
+
+
+def test_ncc_window_1080p_crop_with_near_ties(tv, dev):
+    # the 1920x1080 D=256 generator, cropped so the FP64 oracle stays small
+    L, R, _ = synth.workload_pair(synth.WINDOW_1080P)
+    L, R = np.ascontiguousarray(L[400:490, 1000:1320]), np.ascontiguousarray(R[400:490, 1000:1320])
+    D = 256
+    C64, rho = oracle.cost_ncc(L, R, D, 4, True)
+    got = tv.cost_ncc(to_dev(L, dev), to_dev(R, dev), D, 4, True)
+    _check_ncc(got.cpu().numpy(), C64, rho)
+    # argmax rho (argmin cost): identical except at near-ties of the FP64 costs
+    d_gpu = tv.wta(got).cpu().numpy()
+    d_ref = oracle.wta(C64)
+    srt = np.sort(np.where(np.isfinite(C64), C64, np.inf), axis=2)
+    near = (srt[:, :, 1] - srt[:, :, 0]) < NEAR_TIE
+    assert ((d_gpu == d_ref) | near).all()
+
+
+@pytest.mark.parametrize("W,H,D,flags", [(40, 12, 16, 0), (97, 20, 64, 0), (64, 9, 200, 0),
+                                         (70, 15, 33, 1), (70, 15, 33, 2), (70, 15, 33, 3),
+                                         (1, 5, 8, 0), (130, 7, 256, 0)])
+def test_dp(tv, dev, W, H, D, flags):
+    L, R = pair(W, H, D - 1, 11 * W + D)
+    C = oracle.cost_ssd(L, R, D, 3)
+    want = oracle.dp(C, 20, flags)
+    got = tv.dp(to_dev(C.view(np.int32), dev), 20, flags)
+    assert (got.cpu().numpy() == want).all()
+
+
+def test_ssd_dp_window_1080p(tv, dev):
+    """1920x1080, D = 256: 7x7 SSD (uint32) -> DP (P_occ = 20), bit-exact."""
+    L, R, gt = synth.workload_pair(synth.WINDOW_1080P)
+    C = oracle.cost_ssd(L, R, 256, 3)
+    Ld, Rd = to_dev(L, dev), to_dev(R, dev)
+    Cg = tv.cost_ssd(Ld, Rd, 256, 3)
+    assert (Cg.cpu().numpy().view(np.uint32) == C).all()
+    want = oracle.dp(C, 20)
+    got = tv.dp(Cg, 20).cpu().numpy()
+    assert (got == want).all()
+    assert (np.abs(got.astype(int) - gt) <= 1).mean() > 0.5

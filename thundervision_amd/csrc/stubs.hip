@@ -9,14 +9,6 @@ using namespace tdvh;
         return TDVH_ERR_UNSUPPORTED;                                                                \
     } while (0)
 
-extern "C" int tdvh_cost_ssd(tdvh_ctx *, const uint8_t *, const uint8_t *, int, int, int, int,
-                             tdvh_dsi *, void *) { NOT_YET("tdvh_cost_ssd"); }
-extern "C" int tdvh_cost_ncc(tdvh_ctx *, const uint8_t *, const uint8_t *, int, int, int, int, int,
-                             tdvh_dsi *, void *) { NOT_YET("tdvh_cost_ncc"); }
-extern "C" int tdvh_dp(tdvh_ctx *, const tdvh_dsi *, uint32_t, int, int16_t *, int, void *)
-{ NOT_YET("tdvh_dp"); }
-extern "C" int tdvh_fast_wta_ssd(tdvh_ctx *, const uint8_t *, const uint8_t *, int, int, int, int,
-                                 int, int16_t *, int, void *) { NOT_YET("tdvh_fast_wta_ssd"); }
 extern "C" int tdvh_batch_create(tdvh_ctx *, const tdvh_pair *, int, const tdvh_sgm_params *,
                                  tdvh_batch **) { NOT_YET("tdvh_batch_create"); }
 extern "C" int tdvh_batch_destroy(tdvh_batch *) { NOT_YET("tdvh_batch_destroy"); }
