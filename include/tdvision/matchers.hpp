@@ -1,0 +1,145 @@
+// Stereo matchers and their factory (stereomatcher.hpp:12-25,
+// devstereomatcher.cpp:14-49, fastwtamatcher.cpp:32-52,
+// semigloballabmatcher.cpp:40-68, commonstereomatcherfactory.cpp:24-107).
+#ifndef TDV_MATCHERS_HPP
+#define TDV_MATCHERS_HPP
+
+#include <memory>
+#include <string>
+
+#include "stages.hpp"
+
+TDV_NAMESPACE_BEGIN
+
+class StereoMatcher : public ProcessGroup {
+public:
+    virtual void inputs(ReadPipe<FloatImage> *leftInput, ReadPipe<FloatImage> *rightInput) = 0;
+    virtual ReadPipe<FloatImage> *output() = 0;
+    virtual std::string name() const = 0;
+    virtual Benchmark matchcostBenchmark() const = 0;
+    virtual Benchmark optimizationBenchmark() const = 0;
+};
+
+// Copies optimizer outputs to the host (cpyimagetocpu.cpp:5-10).
+class CpyImageToCPU : public WorkUnit {
+public:
+    CpyImageToCPU() { workName("Copy to CPU"); }
+    void input(ReadPipe<FloatImage> *p) { m_rpipe = p; }
+    ReadPipe<FloatImage> *output() { return &m_wpipe; }
+    bool update();
+
+private:
+    ReadPipe<FloatImage> *m_rpipe = nullptr;
+    ReadWritePipe<FloatImage> m_wpipe;
+};
+
+// Matching cost -> optimizer -> host copy on one device process. The
+// reference's Gaussian prefilter (devstereomatcher.cpp:22-29) is not part of
+// this path; inputs go straight to the cost stage.
+class DevStereoMatcher : public StereoMatcher {
+public:
+    DevStereoMatcher(int device = 0);
+    void inputs(ReadPipe<FloatImage> *leftInput, ReadPipe<FloatImage> *rightInput);
+    Process **processes() { return m_procs; }
+    void setMatchingCost(std::shared_ptr<MatchingCost> mc) { m_matchCost = mc; }
+    void setOptimizer(std::shared_ptr<Optimizer> opt) { m_optimizer = opt; }
+    ReadPipe<FloatImage> *output() { return m_cpy.output(); }
+    std::string name() const;
+    Benchmark matchcostBenchmark() const
+    {
+        return m_matchCost ? m_matchCost->benchmark() : Benchmark();
+    }
+    Benchmark optimizationBenchmark() const
+    {
+        return m_optimizer ? m_optimizer->benchmark() : Benchmark();
+    }
+
+private:
+    std::shared_ptr<MatchingCost> m_matchCost;
+    std::shared_ptr<Optimizer> m_optimizer;
+    CpyImageToCPU m_cpy;
+    DeviceProcess m_process;
+    Process *m_procs[2];
+};
+
+// Base of the fused single-process matchers.
+class FusedMatcher : public StereoMatcher, public Process {
+public:
+    FusedMatcher(int maxDisparity, int device);
+    void inputs(ReadPipe<FloatImage> *l, ReadPipe<FloatImage> *r)
+    {
+        m_lrpipe = l;
+        m_rrpipe = r;
+    }
+    Process **processes() { return m_procs; }
+    ReadPipe<FloatImage> *output() { return &m_wpipe; }
+    Benchmark matchcostBenchmark() const { return m_mark; }
+    Benchmark optimizationBenchmark() const { return m_mark; }
+    void process();
+    bool update();
+
+protected:
+    virtual void run(FloatImage left, FloatImage right, FloatImage out) = 0;
+    int m_maxDisparity;
+    int m_device;
+
+private:
+    ReadPipe<FloatImage> *m_lrpipe = nullptr, *m_rrpipe = nullptr;
+    ReadWritePipe<FloatImage> m_wpipe;
+    Benchmark m_mark;
+    Process *m_procs[2];
+};
+
+// SSD 7x7 + lowest-index WTA without a cost volume.
+class FastWTAMatcher : public FusedMatcher {
+public:
+    explicit FastWTAMatcher(int maxDisparity, int device = 0) : FusedMatcher(maxDisparity, device)
+    {
+    }
+    std::string name() const { return "FastWTAMatcher"; }
+
+protected:
+    void run(FloatImage left, FloatImage right, FloatImage out);
+};
+
+// BT recomputed inside SGM, then WTA: no cost volume in memory.
+class SemiglobalLabMatcher : public FusedMatcher {
+public:
+    explicit SemiglobalLabMatcher(int maxDisparity, int device = 0)
+        : FusedMatcher(maxDisparity, device)
+    {
+    }
+    std::string name() const { return "SemiglobalLabMatcher"; }
+
+protected:
+    void run(FloatImage left, FloatImage right, FloatImage out);
+};
+
+class CommonStereoMatcherFactory {
+public:
+    enum MatchingMode { CPU, Device };
+    enum MatchingCostType { SSD, CrossCorrelationNorm, BirchfieldTomasi };
+    enum OptimizationType { WTA, DynamicProg, SemiGlobal };
+
+    CommonStereoMatcherFactory()
+        : m_mode(Device), m_cost(BirchfieldTomasi), m_opt(SemiGlobal), m_maxDisparity(128)
+    {
+    }
+    void computeDev(MatchingMode m) { m_mode = m; }
+    void matchingCost(MatchingCostType t) { m_cost = t; }
+    void optimization(OptimizationType t) { m_opt = t; }
+    void maxDisparity(int d) { m_maxDisparity = d; }
+    // Throws tdv::Exception for the CPU mode (OpenCV matchers are not part of
+    // this library).
+    StereoMatcher *createStereoMatcher();
+
+private:
+    MatchingMode m_mode;
+    MatchingCostType m_cost;
+    OptimizationType m_opt;
+    int m_maxDisparity;
+};
+
+TDV_NAMESPACE_END
+
+#endif

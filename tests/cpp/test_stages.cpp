@@ -1,0 +1,290 @@
+// Tests of the C++ stage layer. `test_stages cpu` runs the runtime tests
+// (pipes, runner) that need no device, following the reference's
+// testpipeline.cpp (Connections, PipeAndFilter, BoundedBuffer);
+// `test_stages gpu <dir>` runs stage chains on the GPU and compares them with
+// expected results written into <dir> by tests/test_stages_cpp.py.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+#include "tdvision/matchers.hpp"
+
+using namespace tdv;
+
+static int g_fail = 0;
+#define CHECK(c)                                                                                   \
+    do {                                                                                           \
+        if (!(c)) {                                                                                \
+            fprintf(stderr, "CHECK failed %s:%d: %s\n", __FILE__, __LINE__, #c);                   \
+            g_fail++;                                                                              \
+        }                                                                                          \
+    } while (0)
+
+// ---- runtime (CPU) ------------------------------------------------------------
+
+static void test_pipe_connections()
+{
+    ReadWritePipe<int> p(4);
+    p.write(1);
+    p.write(2);
+    int v = 0;
+    CHECK(p.read(&v) && v == 1);
+    CHECK(p.read(&v) && v == 2);
+    p.finish();
+    CHECK(!p.read(&v));
+    {
+        ReadWritePipe<int> q;
+        {
+            WriteGuard<ReadWritePipe<int>> g(q);
+        } // nothing written: the guard finishes the pipe
+        CHECK(!q.read(&v));
+    }
+}
+
+static void test_bounded_buffer()
+{
+    ReadWritePipe<int> p(10);
+    const int N = 1000;
+    std::thread prod([&] {
+        for (int i = 0; i < N; i++) p.write(i);
+        p.finish();
+    });
+    int v, expect = 0;
+    while (p.read(&v)) {
+        CHECK(v == expect);
+        expect++;
+    }
+    prod.join();
+    CHECK(expect == N);
+}
+
+namespace {
+// Doubles its input; ends when the input ends.
+class Doubler : public WorkUnit, public Process {
+public:
+    ReadPipe<int> *in = nullptr;
+    ReadWritePipe<int> out{2};
+    bool update()
+    {
+        WriteGuard<ReadWritePipe<int>> g(out);
+        int v;
+        if (in->read(&v)) g.write(2 * v);
+        return g.wasWrite();
+    }
+    void process()
+    {
+        while (update()) {
+        }
+    }
+};
+class Source : public Process {
+public:
+    ReadWritePipe<int> out{2};
+    int n = 50;
+    void process()
+    {
+        for (int i = 0; i < n; i++) out.write(i);
+        out.finish();
+    }
+};
+class Thrower : public Process {
+public:
+    void process() { throw Exception("planned failure"); }
+};
+class Group : public ProcessGroup {
+public:
+    std::vector<Process *> v;
+    Process **processes()
+    {
+        if (v.empty() || v.back()) v.push_back(nullptr);
+        return v.data();
+    }
+};
+} // namespace
+
+static void test_pipe_and_filter()
+{
+    Source src;
+    Doubler dbl;
+    dbl.in = &src.out;
+    Group g;
+    g.v = {&src, &dbl};
+    ProcessRunner runner(g);
+    runner.run();
+    int v, i = 0;
+    while (dbl.out.read(&v)) {
+        CHECK(v == 2 * i);
+        i++;
+    }
+    runner.join();
+    CHECK(i == 50);
+    CHECK(!runner.errorOcurred());
+}
+
+static void test_runner_reports_errors()
+{
+    Thrower t;
+    Group g;
+    g.v = {&t};
+    ProcessRunner runner(g);
+    runner.run();
+    runner.join();
+    CHECK(runner.errorOcurred());
+    CHECK(runner.lastError().find("planned failure") != std::string::npos);
+}
+
+// ---- GPU stage chains -------------------------------------------------------------
+
+static std::vector<char> slurp(const std::string &path)
+{
+    std::ifstream f(path, std::ios::binary);
+    return std::vector<char>((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+}
+
+struct Case {
+    int W, H, D;
+    std::vector<char> L, R;
+};
+
+// Feeds one pair through a matcher and returns its int16 disparities.
+static std::vector<int16_t> run_matcher(StereoMatcher &m, const Case &c, FloatImage *keep)
+{
+    ReadWritePipe<FloatImage> lp, rp;
+    m.inputs(&lp, &rp);
+    ProcessRunner runner(m);
+    runner.run();
+    lp.write(FloatImage::FromGray8((const uint8_t *)c.L.data(), c.W, c.H));
+    rp.write(FloatImage::FromGray8((const uint8_t *)c.R.data(), c.W, c.H));
+    lp.finish();
+    rp.finish();
+    FloatImage out;
+    std::vector<int16_t> disp(c.W * c.H);
+    if (m.output()->read(&out)) {
+        checkHip(hipMemcpy(disp.data(), out.disparityDev(), disp.size() * 2, hipMemcpyDeviceToHost),
+                 "copy disparity");
+        if (keep) *keep = out;
+    }
+    runner.join();
+    CHECK(!runner.errorOcurred());
+    if (runner.errorOcurred()) fprintf(stderr, "runner: %s\n", runner.lastError().c_str());
+    return disp;
+}
+
+static bool same(const std::vector<int16_t> &a, const std::vector<char> &b, const char *what)
+{
+    const bool ok = a.size() * 2 == b.size() && memcmp(a.data(), b.data(), b.size()) == 0;
+    if (!ok) fprintf(stderr, "%s: disparities differ from the oracle\n", what);
+    return ok;
+}
+
+static void test_gpu(const std::string &dir)
+{
+    Case c;
+    FILE *f = fopen((dir + "/shape.txt").c_str(), "r");
+    CHECK(f && fscanf(f, "%d %d %d", &c.W, &c.H, &c.D) == 3);
+    if (f) fclose(f);
+    c.L = slurp(dir + "/left.u8");
+    c.R = slurp(dir + "/right.u8");
+
+    { // BT + SGM through DevStereoMatcher, plus the float d/D view
+        DevStereoMatcher m;
+        m.setMatchingCost(std::make_shared<BirchfieldCostDev>(c.D));
+        m.setOptimizer(std::make_shared<SemiGlobalDev>());
+        FloatImage out;
+        auto d = run_matcher(m, c, &out);
+        CHECK(same(d, slurp(dir + "/bt_sgm.i16"), "BT+SGM"));
+        const float *fv = out.cpuMem();
+        bool ok = true;
+        for (size_t i = 0; i < d.size(); i++) ok = ok && fv[i] == (float)d[i] / (float)c.D;
+        CHECK(ok);
+        CHECK(m.name() == "Birchfield Cost+Semi-global device");
+        CHECK(m.matchcostBenchmark().probes() == 1 && m.optimizationBenchmark().probes() == 1);
+    }
+    { // fused BT inside SGM
+        SemiglobalLabMatcher m(c.D);
+        CHECK(same(run_matcher(m, c, nullptr), slurp(dir + "/bt_sgm.i16"), "SemiglobalLab"));
+    }
+    { // BT + WTA (lowest index) and tree tie rule
+        DevStereoMatcher m;
+        m.setMatchingCost(std::make_shared<BirchfieldCostDev>(c.D));
+        m.setOptimizer(std::make_shared<WTADev>());
+        CHECK(same(run_matcher(m, c, nullptr), slurp(dir + "/bt_wta.i16"), "BT+WTA"));
+    }
+    { // SSD + DP and SSD + WTA, FastWTAMatcher
+        DevStereoMatcher m;
+        m.setMatchingCost(std::make_shared<SSDDev>(c.D));
+        m.setOptimizer(std::make_shared<DynamicProgDev>(20));
+        CHECK(same(run_matcher(m, c, nullptr), slurp(dir + "/ssd_dp.i16"), "SSD+DP"));
+        FastWTAMatcher fw(c.D);
+        CHECK(same(run_matcher(fw, c, nullptr), slurp(dir + "/ssd_wta.i16"), "FastWTA"));
+    }
+    { // factory
+        CommonStereoMatcherFactory fac;
+        fac.maxDisparity(c.D);
+        fac.matchingCost(CommonStereoMatcherFactory::SSD);
+        fac.optimization(CommonStereoMatcherFactory::WTA);
+        std::unique_ptr<StereoMatcher> m(fac.createStereoMatcher());
+        CHECK(same(run_matcher(*m, c, nullptr), slurp(dir + "/ssd_wta.i16"), "factory SSD+WTA"));
+        fac.computeDev(CommonStereoMatcherFactory::CPU);
+        bool threw = false;
+        try {
+            delete fac.createStereoMatcher();
+        } catch (const Exception &) {
+            threw = true;
+        }
+        CHECK(threw);
+    }
+    { // DSIMem download in the reference layout [x][y][d]
+        ReadWritePipe<FloatImage> lp, rp;
+        BirchfieldCostDev bt(c.D);
+        bt.inputs(&lp, &rp);
+        lp.write(FloatImage::FromGray8((const uint8_t *)c.L.data(), c.W, c.H));
+        rp.write(FloatImage::FromGray8((const uint8_t *)c.R.data(), c.W, c.H));
+        CHECK(bt.update());
+        DSIMem dsi;
+        CHECK(bt.output()->read(&dsi));
+        unsigned char *host = (unsigned char *)dsi.toCpuMem();
+        const auto want = slurp(dir + "/bt_xyd.u16");
+        CHECK(want.size() == dsi.dim().size() * 2 && memcmp(host, want.data(), want.size()) == 0);
+        delete[] host;
+    }
+    { // a device error becomes a tdv::DeviceException
+        bool threw = false;
+        try {
+            checkTdvh(tdvh_sgm(deviceContext(), nullptr, nullptr, nullptr, nullptr, 0, nullptr),
+                      "tdvh_sgm");
+        } catch (const DeviceException &e) {
+            threw = e.status() == TDVH_ERR_INVALID_ARG;
+        }
+        CHECK(threw);
+    }
+}
+
+int main(int argc, char **argv)
+{
+    const std::string mode = argc > 1 ? argv[1] : "cpu";
+    if (mode == "cpu") {
+        test_pipe_connections();
+        test_bounded_buffer();
+        test_pipe_and_filter();
+        test_runner_reports_errors();
+    } else if (mode == "gpu" && argc > 2) {
+        try {
+            test_gpu(argv[2]);
+        } catch (const std::exception &e) {
+            fprintf(stderr, "exception: %s\n", e.what());
+            g_fail++;
+        }
+    } else {
+        fprintf(stderr, "usage: test_stages cpu | gpu <dir>\n");
+        return 2;
+    }
+    printf("%s: %s (%d failures)\n", mode.c_str(), g_fail ? "FAILED" : "ok", g_fail);
+    return g_fail ? 1 : 0;
+}

@@ -1,0 +1,42 @@
+"""Runs the C++ stage-layer tests (tests/cpp/test_stages.cpp, built by
+__graft_entry__.build into thundervision_amd/_bin/test_stages)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from thundervision_amd import synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "thundervision_amd", "_bin", "test_stages")
+
+
+def run(*args):
+    assert os.path.exists(BIN), "build first: python -m thundervision_amd.build"
+    r = subprocess.run([BIN] + list(args), capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def test_runtime_pipes_and_runner():
+    assert "cpu: ok" in run("cpu")
+
+
+@pytest.mark.gpu
+def test_stage_chains_on_gpu(tmp_path):
+    L, R, _ = synth.workload_pair(synth.PLAIN_384)
+    D = 16
+    H, W = L.shape
+    (tmp_path / "shape.txt").write_text("%d %d %d\n" % (W, H, D))
+    L.tofile(tmp_path / "left.u8")
+    R.tofile(tmp_path / "right.u8")
+    C = oracle.cost_bt2(L, R, D)
+    oracle.wta(oracle.sgm_std(C, 8, 32)).tofile(tmp_path / "bt_sgm.i16")
+    oracle.wta(C).tofile(tmp_path / "bt_wta.i16")
+    np.ascontiguousarray(C.transpose(1, 0, 2)).tofile(tmp_path / "bt_xyd.u16")
+    S = oracle.cost_ssd(L, R, D, 3)
+    oracle.dp(S, 20).tofile(tmp_path / "ssd_dp.i16")
+    oracle.wta(S).tofile(tmp_path / "ssd_wta.i16")
+    assert "gpu: ok" in run("gpu", str(tmp_path))

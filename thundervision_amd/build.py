@@ -2,6 +2,8 @@
 
 libtdvh.so     HIP kernels + C ABI (include/tdvh.h), gfx950, hipcc
 libtdvsynth.so seeded synthetic stereo-pair generator (plain C, gcc)
+libtdvision.so C++ stage layer (include/tdvision/*.hpp) over libtdvh.so
+_bin/test_stages  C++ tests of the stage layer (run by tests/test_stages_cpp.py)
 """
 import os
 import subprocess
@@ -50,6 +52,22 @@ def build(verbose=False):
     lib = os.path.join(PKG, "libtdvh.so")
     _run([HIPCC, "-shared", "--offload-arch=gfx950", "-o", lib + ".tmp"] + objs)
     os.replace(lib + ".tmp", lib)
+    # C++ stage layer: host-only code over the C ABI
+    stage_srcs = [os.path.join(CSRC, "stages", f)
+                  for f in ("runtime.cpp", "images.cpp", "stages.cpp", "matchers.cpp")]
+    cxx = ["g++", "-O2", "-std=c++17", "-fPIC", "-D__HIP_PLATFORM_AMD__",
+           "-I" + INCLUDE, "-I/opt/rocm/include"]
+    stlib = os.path.join(PKG, "libtdvision.so")
+    _run(cxx + ["-shared", "-o", stlib + ".tmp"] + stage_srcs +
+         ["-L" + PKG, "-ltdvh", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN",
+          "-Wl,-rpath,/opt/rocm/lib", "-pthread"])
+    os.replace(stlib + ".tmp", stlib)
+    os.makedirs(os.path.join(PKG, "_bin"), exist_ok=True)
+    tbin = os.path.join(PKG, "_bin", "test_stages")
+    _run(cxx + ["-o", tbin + ".tmp", os.path.join(ROOT, "tests", "cpp", "test_stages.cpp"),
+                "-L" + PKG, "-ltdvision", "-ltdvh", "-L/opt/rocm/lib", "-lamdhip64",
+                "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath,/opt/rocm/lib", "-pthread"])
+    os.replace(tbin + ".tmp", tbin)
     synth = os.path.join(PKG, "libtdvsynth.so")
     _run(["gcc", "-O2", "-shared", "-fPIC", "-o", synth + ".tmp",
           os.path.join(CSRC, "synth.c"), "-lm"])

@@ -1,0 +1,122 @@
+// Stereo matchers and the factory.
+#include <hip/hip_runtime.h>
+
+#include "tdvision/matchers.hpp"
+
+TDV_NAMESPACE_BEGIN
+
+bool CpyImageToCPU::update()
+{
+    WriteGuard<ReadWritePipe<FloatImage>> guard(m_wpipe);
+    FloatImage img;
+    if (m_rpipe->read(&img)) {
+        img.cpuMem();
+        guard.write(img);
+    }
+    return guard.wasWrite();
+}
+
+DevStereoMatcher::DevStereoMatcher(int device) : m_process(device)
+{
+    m_procs[0] = &m_process;
+    m_procs[1] = nullptr;
+}
+
+void DevStereoMatcher::inputs(ReadPipe<FloatImage> *leftInput, ReadPipe<FloatImage> *rightInput)
+{
+    if (!m_matchCost || !m_optimizer) throw Exception("DevStereoMatcher: stages not set");
+    m_matchCost->inputs(leftInput, rightInput);
+    m_optimizer->input(m_matchCost->output());
+    m_cpy.input(m_optimizer->output());
+    m_process.addWork(m_matchCost.get());
+    m_process.addWork(m_optimizer.get());
+    m_process.addWork(&m_cpy);
+}
+
+std::string DevStereoMatcher::name() const
+{
+    return (m_matchCost ? m_matchCost->workName() : std::string("")) + "+" +
+           (m_optimizer ? m_optimizer->workName() : std::string(""));
+}
+
+FusedMatcher::FusedMatcher(int maxDisparity, int device)
+    : m_maxDisparity(maxDisparity), m_device(device)
+{
+    m_procs[0] = this;
+    m_procs[1] = nullptr;
+}
+
+void FusedMatcher::process()
+{
+    checkHip(hipSetDevice(m_device), "hipSetDevice");
+    while (update()) {
+    }
+}
+
+bool FusedMatcher::update()
+{
+    WriteGuard<ReadWritePipe<FloatImage>> guard(m_wpipe);
+    FloatImage left, right;
+    if (m_lrpipe->read(&left) && m_rrpipe->read(&right)) {
+        if (left.dim() != right.dim()) throw Exception("left and right images differ in size");
+        FloatImage out = FloatImage::CreateDev(left.dim());
+        out.disparityDepth(m_maxDisparity);
+        DeviceBenchmarker bm;
+        bm.begin();
+        run(left, right, out);
+        out.syncFromDisparity();
+        bm.end();
+        m_mark.addProbe(bm.elapsedTime());
+        out.cpuMem();
+        guard.write(out);
+    }
+    return guard.wasWrite();
+}
+
+void FastWTAMatcher::run(FloatImage left, FloatImage right, FloatImage out)
+{
+    const int W = (int)left.dim().width(), H = (int)left.dim().height();
+    checkTdvh(tdvh_fast_wta_ssd(deviceContext(m_device), left.devMemU8(), right.devMemU8(), W, H,
+                                W, m_maxDisparity, 3, out.disparityDev(), W, nullptr),
+              "tdvh_fast_wta_ssd");
+}
+
+void SemiglobalLabMatcher::run(FloatImage left, FloatImage right, FloatImage out)
+{
+    const int W = (int)left.dim().width(), H = (int)left.dim().height();
+    const tdvh_sgm_params p{8, 32, 0xFF, 0};
+    checkTdvh(tdvh_stereo_bt_sgm(deviceContext(m_device), left.devMemU8(), right.devMemU8(), W, H,
+                                 W, m_maxDisparity, &p, TDVH_FUSE_COST, out.disparityDev(), W,
+                                 nullptr),
+              "tdvh_stereo_bt_sgm");
+}
+
+StereoMatcher *CommonStereoMatcherFactory::createStereoMatcher()
+{
+    if (m_mode == CPU)
+        throw Exception("CPU (OpenCV) matchers are not part of this library; use Device");
+    std::shared_ptr<MatchingCost> cost;
+    switch (m_cost) {
+    case SSD: cost = std::make_shared<SSDDev>(m_maxDisparity); break;
+    case CrossCorrelationNorm: cost = std::make_shared<CrossCorrelationDev>(m_maxDisparity); break;
+    case BirchfieldTomasi: cost = std::make_shared<BirchfieldCostDev>(m_maxDisparity); break;
+    }
+    std::shared_ptr<Optimizer> opt;
+    switch (m_opt) {
+    case WTA: opt = std::make_shared<WTADev>(); break;
+    case DynamicProg:
+        if (m_cost != SSD) throw Exception("DynamicProgDev runs on uint32 SSD costs");
+        opt = std::make_shared<DynamicProgDev>();
+        break;
+    case SemiGlobal:
+        if (m_cost != BirchfieldTomasi) throw Exception("SemiGlobalDev runs on uint16 BT costs");
+        opt = std::make_shared<SemiGlobalDev>();
+        break;
+    }
+    DevStereoMatcher *m = new DevStereoMatcher(0);
+    m->setMatchingCost(cost);
+    m->setOptimizer(opt);
+    return m;
+}
+
+TDV_NAMESPACE_END
