@@ -240,6 +240,9 @@ int tdvh_batch_config(tdvh_batch *b, int lines_per_wave, int waves_per_group,
 /* Enqueues the whole batch on `stream`. With use_graph != 0 the launch
  * sequence is captured once into a hipGraph and replayed. */
 int tdvh_batch_run(tdvh_batch *b, int use_graph, void *stream);
+/* Synchronises `stream` and reports a stalled work queue (a dependency wait
+ * that hit its spin limit) as TDVH_ERR_DEVICE. */
+int tdvh_batch_wait(tdvh_batch *b, void *stream);
 /* Total W*H*D cells of the batch. */
 int64_t tdvh_batch_cells(const tdvh_batch *b);
 /* Device bytes held by the batch. */

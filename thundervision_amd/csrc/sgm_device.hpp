@@ -1,0 +1,405 @@
+// Device code of the standard-geometry SGM (shared by the per-pair kernels
+// in sgm.hip and the persistent batch kernel in batch.hip).
+#pragma once
+
+#include "kernels.hpp"
+
+namespace tdvh {
+
+
+struct LineSet {
+    int prio; // raise the issue priority of waves scanning long lines
+    int ndirs;
+    int dir[8];
+    int wave_begin[9]; // prefix sums of waves (4 lines per wave) per slot
+};
+
+// Start pixel and length of standard-geometry line `line` of direction `dir`.
+__host__ __device__ inline void line_geom(int dir, int line, int W, int H, int &sx, int &sy,
+                                          int &len)
+{
+    switch (dir) {
+    case 0: sx = 0; sy = line; len = W; break;
+    case 1: sx = W - 1; sy = line; len = W; break;
+    case 2: sx = line; sy = 0; len = H; break;
+    case 3: sx = line; sy = H - 1; len = H; break;
+    case 4: // (1,1): top row, then left column
+        if (line < W) { sx = line; sy = 0; } else { sx = 0; sy = line - W + 1; }
+        len = min(W - sx, H - sy);
+        break;
+    case 5: // (-1,-1): bottom row, then right column
+        if (line < W) { sx = line; sy = H - 1; } else { sx = W - 1; sy = H - 2 - (line - W); }
+        len = min(sx + 1, sy + 1);
+        break;
+    case 6: // (-1,1): top row, then right column
+        if (line < W) { sx = line; sy = 0; } else { sx = W - 1; sy = line - W + 1; }
+        len = min(sx + 1, H - sy);
+        break;
+    default: // 7, (1,-1): bottom row, then left column
+        if (line < W) { sx = line; sy = H - 1; } else { sx = 0; sy = H - 2 - (line - W); }
+        len = min(W - sx, sy + 1);
+        break;
+    }
+}
+
+// Device-side inputs of the SGM kernels.
+struct SgmArgs {
+    int W, H, D, P1, P2;
+    const uint32_t *C;  // SPLIT16 cost volume (staged), or null
+    const uint32_t *LT; // 3 planes W*H: splatted left BT terms l2, lmin, lmax (fused)
+    const uint32_t *RT; // 3 planes H*Wr: paired right BT terms r2, rmin, rmax (fused)
+    int Wr;             // row length of RT: W + 16m
+    uint8_t *planes;    // path-cost planes (E bytes or L words)
+    size_t plane_bytes;
+    uint8_t *trash;     // >= 64 * 64 bytes: stores of lanes past their line
+};
+
+// Packed uint16 saturating subtract / max.
+__device__ __forceinline__ uint32_t pk_subs(uint32_t a, uint32_t b)
+{
+    return u32(__builtin_elementwise_sub_sat(v2(a), v2(b)));
+}
+__device__ __forceinline__ uint32_t pk_max(uint32_t a, uint32_t b)
+{
+    return u32(__builtin_elementwise_max(v2(a), v2(b)));
+}
+
+// Invalid-cell mask of lane j, word r: d >= D (padding) or d > x.
+template <int M>
+__device__ __forceinline__ uint32_t cell_mask(int j, int r, int D, int x)
+{
+    const int dlo = j * M + r, dhi = 16 * M + j * M + r;
+    return ((dlo >= D || dlo > x) ? 0x0000FFFFu : 0u) | ((dhi >= D || dhi > x) ? 0xFFFF0000u : 0u);
+}
+
+// BT2 costs of the 2M cells a lane owns at pixel (x, y). lt: splatted left
+// terms of the pixel; the right terms come as words i0 + r of the paired
+// planes (word i holds the terms of right pixels W-1-i and W-1-i-16m).
+template <int M>
+__device__ __forceinline__ void bt_costs(const uint32_t (&lt)[3], const uint32_t (&r2)[M],
+                                         const uint32_t (&rmin)[M], const uint32_t (&rmax)[M],
+                                         int j, int D, int x, uint32_t (&c)[M])
+{
+    const bool edge = x < 32 * M - 1 || D < 32 * M; // some cell of the vector is invalid
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        const uint32_t a = pk_max(pk_subs(lt[0], rmax[r]), pk_subs(rmin[r], lt[0]));
+        const uint32_t b = pk_max(pk_subs(r2[r], lt[2]), pk_subs(lt[1], r2[r]));
+        c[r] = pk_min(a, b);
+        if (edge) c[r] |= cell_mask<M>(j, r, D, x);
+    }
+}
+
+// One SGM step on the packed vector of a line. Lp: previous L (in/out),
+// c: costs of the current pixel, m: previous minimum (in/out); E receives
+// L - C = min(...) - m, in [0, P2] for valid cells.
+template <int M>
+__device__ __forceinline__ void sgm_step(uint32_t (&Lp)[M], const uint32_t (&c)[M], uint32_t &m,
+                                         int j, uint32_t p1, uint32_t P2, uint32_t (&E)[M])
+{
+    uint32_t A = row_ror1(Lp[M - 1]);
+    if (j == 0) A = (A << 16) | 0xFFFFu; // d-1 of d=0 is absent; of d=16m is lane 15's low
+    uint32_t B = row_ror15(Lp[0]);
+    if (j == 15) B = (B >> 16) | 0xFFFF0000u; // d+1 of d=16m-1 is lane 0's high; of Dp-1 absent
+    const uint32_t mm = pk_splat(m), mp2 = pk_splat(m + P2);
+    uint32_t nmin = 0xFFFFFFFFu;
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        const uint32_t dm1 = r == 0 ? A : Lp[r - 1];
+        const uint32_t dp1 = r == M - 1 ? B : Lp[r + 1];
+        uint32_t t = pk_adds(pk_min(dm1, dp1), p1);
+        t = pk_min(pk_min(t, Lp[r]), mp2);
+        E[r] = pk_sub(t, mm);
+    }
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        Lp[r] = pk_adds(E[r], c[r]);
+        nmin = pk_min(nmin, Lp[r]);
+    }
+    m = row_min_u32(min(nmin & 0xFFFFu, nmin >> 16));
+}
+
+// Low bytes of the two halves of words a (-> bytes 0,1) and b (-> bytes 2,3).
+__device__ __forceinline__ uint32_t pack_e(uint32_t a, uint32_t b)
+{
+    return __builtin_amdgcn_perm(b, a, 0x06040200u);
+}
+
+// One wave scans lines line0 .. line0+3 of direction `dir` and writes its
+// path-cost plane `s`. Called by the per-pair kernel and the batch kernel.
+template <int M, bool FUSED, bool EOUT, int PF>
+__device__ __forceinline__ void sgm_line_group(const SgmArgs &a, int dir, int line0, int s,
+                                               int prio)
+{
+    constexpr int WPP = 16 * M;
+    constexpr int NW = FUSED ? 3 + 3 * M : M; // prefetched words per step
+    const int lane = threadIdx.x & 63;
+    const int line = line0 + (lane >> 4);
+    const int j = lane & 15;
+    const int W = a.W, H = a.H;
+    // Rows past the last line keep a one-pixel dummy line at (0, 0) so that
+    // every lane runs the same, wave-uniform loop; they never store.
+    int sx = 0, sy = 0, len = 0;
+    if (line < dir_lines(dir, W, H)) line_geom(dir, line, W, H, sx, sy, len);
+    const int dx = dir_dx(dir), dy = dir_dy(dir);
+    const long step = (long)dy * W + dx;
+    const long p0 = (long)sy * W + sx;
+    const int klast = len > 0 ? len - 1 : 0;
+    // wave-uniform trip count: the longest of the wave's 4 lines
+    const int kmax = max(max(__builtin_amdgcn_readlane(len, 0), __builtin_amdgcn_readlane(len, 16)),
+                         max(__builtin_amdgcn_readlane(len, 32), __builtin_amdgcn_readlane(len, 48)));
+    uint8_t *outp = a.planes + (size_t)s * a.plane_bytes;
+    const size_t lt_plane = (size_t)W * H, rt_plane = (size_t)a.Wr * H;
+    // Long lines set the length of the launch: let their waves issue first
+    // on a shared SIMD (instruction-issue priority within the CU only).
+    if (prio) {
+        const int longest = max(W, H);
+        if (kmax * 4 >= longest * 3) __builtin_amdgcn_s_setprio(3);
+        else if (kmax * 2 >= longest) __builtin_amdgcn_s_setprio(2);
+        else if (kmax * 4 >= longest) __builtin_amdgcn_s_setprio(1);
+    }
+
+    uint32_t ring[PF][NW];
+    // Loads of step k (clamped to the line, so always in bounds); no branch.
+    auto fetch = [&](uint32_t(&slot)[NW], int k) {
+        const int kk = k < klast ? k : klast;
+        const long p = p0 + kk * step;
+        if constexpr (!FUSED) {
+            const uint32_t *src = a.C + (size_t)p * WPP + j * M;
+#pragma unroll
+            for (int r = 0; r < M; r++) slot[r] = src[r];
+        } else {
+            const int x = sx + kk * dx, y = sy + kk * dy;
+#pragma unroll
+            for (int t = 0; t < 3; t++) slot[t] = a.LT[t * lt_plane + p];
+            const uint32_t *rt = a.RT + (size_t)y * a.Wr + (W - 1 - x) + j * M;
+#pragma unroll
+            for (int t = 0; t < 3; t++)
+#pragma unroll
+                for (int r = 0; r < M; r++) slot[3 + t * M + r] = rt[t * rt_plane + r];
+        }
+    };
+
+#pragma unroll
+    for (int u = 0; u < PF; u++) fetch(ring[u], u);
+
+    // L' = 0 and m = 0 make the first step return L = C (path start).
+    uint32_t Lp[M];
+#pragma unroll
+    for (int r = 0; r < M; r++) Lp[r] = 0;
+    uint32_t m = 0;
+    const uint32_t p1 = pk_splat(a.P1);
+    uint32_t padm[M];
+#pragma unroll
+    for (int r = 0; r < M; r++) padm[r] = cell_mask<M>(j, r, a.D, 1 << 30);
+    const bool any_pad = a.D < 32 * M;
+
+    // One step: costs from ring slot u, refill the slot, recurrence, store.
+    // Every lane issues the same loads and stores (lanes past their line
+    // store into a trash buffer), so the in-order memory counters stay exact
+    // and the waits only cover the slot being consumed.
+    auto body = [&](uint32_t(&slot)[NW], int k) {
+        uint32_t c[M];
+        const int kc = k < klast ? k : klast;
+        if constexpr (!FUSED) {
+#pragma unroll
+            for (int r = 0; r < M; r++) c[r] = slot[r] | padm[r];
+        } else {
+            const int x = sx + kc * dx;
+#pragma unroll
+            for (int r = 0; r < M; r++) {
+                const uint32_t l2 = slot[0], lmn = slot[1], lmx = slot[2];
+                const uint32_t r2 = slot[3 + r], rmn = slot[3 + M + r], rmx = slot[3 + 2 * M + r];
+                const uint32_t aa = pk_max(pk_subs(l2, rmx), pk_subs(rmn, l2));
+                const uint32_t bb = pk_max(pk_subs(r2, lmx), pk_subs(lmn, r2));
+                c[r] = pk_min(aa, bb);
+            }
+            // cells with d > x exist only near the left border
+            const bool near_left = x < 32 * M - 1;
+            if (__builtin_amdgcn_ballot_w64(near_left) != 0 || any_pad) {
+#pragma unroll
+                for (int r = 0; r < M; r++) c[r] |= cell_mask<M>(j, r, a.D, x);
+            }
+        }
+        fetch(slot, k + PF);
+        uint32_t E[M];
+        sgm_step<M>(Lp, c, m, j, p1, (uint32_t)a.P2, E);
+        const size_t p = (size_t)(p0 + (long)kc * step);
+        if constexpr (EOUT) {
+            // E in bytes: lane j holds 2M bytes at (p*32M + j*2M)
+            uint8_t *dst = k < len ? outp + p * (32 * M) + j * (2 * M) : a.trash + lane * (2 * M);
+            if constexpr (M % 2 == 0) {
+#pragma unroll
+                for (int r = 0; r < M; r += 2) ((uint32_t *)dst)[r / 2] = pack_e(E[r], E[r + 1]);
+            } else {
+#pragma unroll
+                for (int r = 0; r < M; r++) ((uint16_t *)dst)[r] = (uint16_t)pack_e(E[r], 0);
+            }
+        } else {
+            uint32_t *dst = k < len ? (uint32_t *)outp + p * WPP + j * M
+                                    : (uint32_t *)a.trash + lane * M;
+#pragma unroll
+            for (int r = 0; r < M; r++) dst[r] = Lp[r];
+        }
+    };
+
+    int k0 = 0;
+    for (; k0 + PF <= kmax; k0 += PF) {
+#pragma unroll
+        for (int u = 0; u < PF; u++) body(ring[u], k0 + u);
+    }
+#pragma unroll
+    for (int u = 0; u < PF; u++)
+        if (k0 + u < kmax) body(ring[u], k0 + u);
+}
+
+template <int M, bool FUSED, bool EOUT, int PF>
+__global__ __launch_bounds__(256) void sgm_lines_kernel(const SgmArgs a, const LineSet ls)
+{
+    const int gw = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (gw >= ls.wave_begin[ls.ndirs]) return;
+    int s = 0;
+    while (s < ls.ndirs - 1 && gw >= ls.wave_begin[s + 1]) s++;
+    sgm_line_group<M, FUSED, EOUT, PF>(a, ls.dir[s], (gw - ls.wave_begin[s]) * 4, s, ls.prio);
+}
+
+// ---- BT term planes for the fused path ---------------------------------------
+
+// LT[t][y][x]: splatted left terms; RT[t][y][i]: right terms of pixels
+// W-1-i (low half) and W-1-i-16m (high half), clamped at x' = 0.
+__device__ __forceinline__ void bt_planes_at(const uint8_t *__restrict__ L,
+                                             const uint8_t *__restrict__ R, int W, int H,
+                                             int pitch, int m, uint32_t *__restrict__ LT,
+                                             uint32_t *__restrict__ RT, int Wr, int i, int y)
+{
+    const uint8_t *lr = L + (size_t)y * pitch;
+    const uint8_t *rr = R + (size_t)y * pitch;
+    auto terms = [&](const uint8_t *row, int x, int &v2, int &mn, int &mx) {
+        x = x < 0 ? 0 : x;
+        const int c = row[x];
+        const int a = c + row[x > 0 ? x - 1 : 0], b = c + row[x < W - 1 ? x + 1 : W - 1];
+        v2 = 2 * c;
+        mn = min(v2, min(a, b));
+        mx = max(v2, max(a, b));
+    };
+    if (i < W) {
+        int v2, mn, mx;
+        terms(lr, i, v2, mn, mx);
+        const size_t o = (size_t)y * W + i, pl = (size_t)W * H;
+        LT[o] = pk_splat(v2);
+        LT[pl + o] = pk_splat(mn);
+        LT[2 * pl + o] = pk_splat(mx);
+    }
+    if (i < Wr) {
+        int a2, amn, amx, b2, bmn, bmx;
+        const int xa = W - 1 - i;
+        terms(rr, xa, a2, amn, amx);
+        terms(rr, xa - 16 * m, b2, bmn, bmx);
+        const size_t o = (size_t)y * Wr + i, pl = (size_t)Wr * H;
+        RT[o] = (uint32_t)a2 | ((uint32_t)b2 << 16);
+        RT[pl + o] = (uint32_t)amn | ((uint32_t)bmn << 16);
+        RT[2 * pl + o] = (uint32_t)amx | ((uint32_t)bmx << 16);
+    }
+}
+
+
+// ---- plane sum + lowest-index WTA ----------------------------------------
+
+// Unpacks the two bytes of half h of word w into a packed uint16x2.
+__device__ __forceinline__ uint32_t unpack_e(uint32_t w, int h)
+{
+    return __builtin_amdgcn_perm(0u, w, h ? 0x0C030C02u : 0x0C010C00u);
+}
+
+// Sum of the planes and lowest-index argmin of pixel p by one 16-lane row.
+template <int M, bool FUSED, bool EOUT>
+__device__ __forceinline__ void sgm_sum_pixel(const SgmArgs &a, int nplanes,
+                                              uint32_t *__restrict__ S,
+                                              int16_t *__restrict__ disp, int pitch, long p)
+{
+    constexpr int WPP = 16 * M;
+    const int W = a.W, H = a.H;
+    const int j = threadIdx.x & 15;
+    const int y = (int)(p / W), x = (int)(p - (long)y * W);
+    uint32_t acc[M];
+    if constexpr (EOUT) {
+        // S = n*C + sum of E (invalid where C is)
+        uint32_t c[M];
+        if constexpr (FUSED) {
+            const size_t lt_plane = (size_t)W * H, rt_plane = (size_t)a.Wr * H;
+            const uint32_t lt[3] = {a.LT[p], a.LT[lt_plane + p], a.LT[2 * lt_plane + p]};
+            const uint32_t *rt = a.RT + (size_t)y * a.Wr + (W - 1 - x) + j * M;
+            uint32_t r2[M], rmn[M], rmx[M];
+#pragma unroll
+            for (int r = 0; r < M; r++) {
+                r2[r] = rt[r];
+                rmn[r] = rt[rt_plane + r];
+                rmx[r] = rt[2 * rt_plane + r];
+            }
+            bt_costs<M>(lt, r2, rmn, rmx, j, a.D, x, c);
+        } else {
+            const uint32_t *src = a.C + (size_t)p * WPP + j * M;
+#pragma unroll
+            for (int r = 0; r < M; r++) c[r] = src[r] | cell_mask<M>(j, r, a.D, 1 << 30);
+        }
+#pragma unroll
+        for (int r = 0; r < M; r++) acc[r] = 0;
+        for (int q = 0; q < nplanes; q++) {
+            const uint8_t *src = a.planes + q * a.plane_bytes + (size_t)p * (32 * M) + j * (2 * M);
+            if constexpr (M % 2 == 0) {
+#pragma unroll
+                for (int r = 0; r < M; r += 2) {
+                    const uint32_t w = ((const uint32_t *)src)[r / 2];
+                    acc[r] = pk_add(acc[r], unpack_e(w, 0));
+                    acc[r + 1] = pk_add(acc[r + 1], unpack_e(w, 1));
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < M; r++)
+                    acc[r] = pk_add(acc[r], unpack_e(((const uint16_t *)src)[r], 0));
+            }
+        }
+        const uint32_t n = pk_splat((uint32_t)nplanes);
+#pragma unroll
+        for (int r = 0; r < M; r++) {
+            const uint32_t inv = pk_sub(0u, u32(v2(c[r]) >> (tdvh_u16x2)(15))); // 0xFFFF where C invalid
+            acc[r] = pk_add(u32(v2(c[r]) * v2(n)), acc[r]) | inv;
+        }
+    } else {
+        const size_t base = (size_t)p * WPP + j * M;
+        const uint32_t *L = (const uint32_t *)a.planes;
+        const size_t plane = a.plane_bytes / 4;
+#pragma unroll
+        for (int r = 0; r < M; r++) acc[r] = L[base + r];
+        for (int q = 1; q < nplanes; q++) {
+            const uint32_t *src = L + q * plane + base;
+#pragma unroll
+            for (int r = 0; r < M; r++) acc[r] = pk_adds(acc[r], src[r]);
+        }
+    }
+    if (S) {
+#pragma unroll
+        for (int r = 0; r < M; r++) S[(size_t)p * WPP + j * M + r] = acc[r];
+    }
+    uint32_t key = 0xFFFFFFFFu;
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        const uint32_t dlo = j * M + r, dhi = 16 * M + j * M + r;
+        key = min(key, ((acc[r] & 0xFFFFu) << 16) | dlo);
+        key = min(key, (acc[r] & 0xFFFF0000u) | dhi);
+    }
+    key = row_min_u32(key);
+    if (j == 0) disp[(size_t)y * pitch + x] = (int16_t)(key & 0xFFFFu);
+}
+
+template <int M, bool FUSED, bool EOUT>
+__global__ __launch_bounds__(256) void sgm_sum_wta_kernel(const SgmArgs a, int nplanes,
+                                                          uint32_t *__restrict__ S,
+                                                          int16_t *__restrict__ disp, int pitch)
+{
+    const long p = (long)blockIdx.x * 16 + (threadIdx.x >> 4);
+    if (p >= (long)a.W * a.H) return;
+    sgm_sum_pixel<M, FUSED, EOUT>(a, nplanes, S, disp, pitch, p);
+}
+
+} // namespace tdvh

@@ -86,6 +86,7 @@ SIGNATURES = {
     "tdvh_batch_destroy": (_I, [_VP]),
     "tdvh_batch_config": (_I, [_VP, _I, _I, _I]),
     "tdvh_batch_run": (_I, [_VP, _I, _VP]),
+    "tdvh_batch_wait": (_I, [_VP, _VP]),
     "tdvh_batch_cells": (_I64, [_VP]),
     "tdvh_batch_scratch_bytes": (_SZ, [_VP]),
 }
@@ -366,6 +367,10 @@ class Batch:
 
     def run(self, use_graph=True, stream=None):
         _check(lib().tdvh_batch_run(self.handle, int(use_graph), _stream(stream)))
+
+    def wait(self, stream=None):
+        """Synchronises and raises if the work queue stalled."""
+        _check(lib().tdvh_batch_wait(self.handle, _stream(stream)))
 
     def cells(self):
         return lib().tdvh_batch_cells(self.handle)
