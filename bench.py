@@ -7,6 +7,8 @@ is one pass of the path over that pair with the images already in HBM.
 Prints one JSON line.
 
   python bench.py [--steps K] [--warmup W] [--mode fused|staged]
+  python bench.py --profile-run          a few plain steps only (for rocprofv3)
+  python bench.py --copy-calibration N   a plain N-byte device copy (PMC scale)
 """
 import argparse
 import json
@@ -38,6 +40,52 @@ def cpu_model():
     return "unknown"
 
 
+def bench_batched(tv, synth, torch, ctx, reps):
+    """64 mixed pairs (320x240..1920x1080, D in {32,64,128,256} clamped to
+    W/4, seeds 1000+i): one persistent launch replayed from a hipGraph,
+    against the plain per-pair call over the same pairs."""
+    from thundervision_amd import autotune
+    specs = synth.batch_specs(64)
+    dev = torch.device("cuda:0")
+    pairs = []
+    for s in specs:
+        L, R, _ = synth.pair(s["width"], s["height"], s["dmax"], s["nrect"], s["seed"])
+        pairs.append((torch.from_numpy(L).to(dev), torch.from_numpy(R).to(dev), s["depth"],
+                      torch.empty((s["height"], s["width"]), dtype=torch.int16, device=dev)))
+    cells = sum(s["width"] * s["height"] * s["depth"] for s in specs)
+    b = tv.Batch(pairs, ctx=ctx)
+    tuned = autotune.tune(b, [(s["width"], s["height"], s["depth"]) for s in specs], reps=2)
+    ms_graph = autotune.time_batch(b, reps, use_graph=True)
+    # plain per-pair calls over the same pairs
+    def plain_round():
+        for L, R, D, disp in pairs:
+            tv.stereo_bt_sgm(L, R, D, out=disp, ctx=ctx)
+    plain_round()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plain_round()
+    torch.cuda.synchronize()
+    ms_plain = (time.perf_counter() - t0) / reps * 1e3
+    # bit-exact check of the last batch run against the plain outputs
+    b.run(use_graph=True)
+    b.wait()
+    outs = [d.clone() for _, _, _, d in pairs]
+    plain_round()
+    torch.cuda.synchronize()
+    exact = all(torch.equal(o, d) for o, (_, _, _, d) in zip(outs, pairs))
+    return {"workload": "64 pairs, sizes {320x240,640x480,1280x720,1920x1080}, D in "
+                        "{32,64,128,256} clamped to W/4, seeds 1000+i; BT+SGM8+WTA",
+            "cells": cells, "ms_per_batch": round(ms_graph, 3),
+            "value": round(cells / (ms_graph * 1e-3) / 1e6, 1), "unit": "MDE/s",
+            "plain_per_pair_ms": round(ms_plain, 3),
+            "plain_per_pair_value": round(cells / (ms_plain * 1e-3) / 1e6, 1),
+            "speedup_vs_plain": round(ms_plain / ms_graph, 3),
+            "tuned": {"waves_per_group": tuned["waves_per_group"], "prefetch": tuned["prefetch"],
+                      "cached": tuned["cached"]},
+            "bit_exact_vs_plain": exact}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -45,6 +93,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--mode", choices=["fused", "staged"], default="fused")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--profile-run", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every pair and stream is independent, so N GPUs "
@@ -56,6 +107,16 @@ def main():
 
     import thundervision_amd.tdvh as tv
     from thundervision_amd import synth
+
+    if args.copy_calibration:
+        n = args.copy_calibration
+        src = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n, "launches": 3}))
+        return
 
     spec = synth.PLAIN_720P
     W, H, D = spec["width"], spec["height"], spec["depth"]
@@ -82,6 +143,12 @@ def main():
         return (time.perf_counter() - t0) / steps
 
     cells = W * H * D
+    if args.profile_run:
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_run": args.mode, "steps": args.steps}))
+        return
     sec = timed(fusion, args.steps, args.warmup)
     value = cells / sec / 1e6
     other = tv.FUSE_STAGED if fusion == tv.FUSE_COST else tv.FUSE_COST
@@ -99,6 +166,10 @@ def main():
     dom_ms = table[dom][0] / table[dom][1]
     achieved = b_alg(W, H, D) / (dom_ms * 1e-3) / 1e9
     pipeline_ms = sum(v[0] / v[1] for v in table.values())
+
+    batched = None
+    if not args.no_batched:
+        batched = bench_batched(tv, synth, torch, ctx, max(3, args.steps // 5))
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -143,6 +214,7 @@ def main():
                        "value": round(cells / sec_other / 1e6, 1),
                        "ms_per_step": round(sec_other * 1e3, 4)},
         "kernels_ms": {k: round(v[0] / v[1], 4) for k, v in table.items()},
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

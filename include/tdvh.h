@@ -215,11 +215,12 @@ typedef struct {
  * when not NULL. */
 size_t tdvh_sgm_paths(int width, int height, tdvh_sgpath *paths);
 
-/* ---- batched persistent execution ------------------------------------- */
-/* Many pairs of uneven size and disparity range in one persistent launch
- * fed by a device work queue of (pair, direction, line-group) tiles.
- * There is no counterpart in the reference; it runs the same arithmetic as
- * tdvh_stereo_bt_sgm on every pair. */
+/* ---- batched execution ------------------------------------------------- */
+/* Many pairs of uneven size and disparity range planned once into device
+ * work-item lists (BT rows, 4-line SGM groups ordered longest first, pixel
+ * chunks) and run as one captured hipGraph whose depth classes are parallel
+ * branches. There is no counterpart in the reference; it runs the same
+ * arithmetic as tdvh_stereo_bt_sgm on every pair. */
 typedef struct {
     const uint8_t *left, *right; /* device */
     int32_t width, height, pitch, depth;
@@ -240,8 +241,7 @@ int tdvh_batch_config(tdvh_batch *b, int lines_per_wave, int waves_per_group,
 /* Enqueues the whole batch on `stream`. With use_graph != 0 the launch
  * sequence is captured once into a hipGraph and replayed. */
 int tdvh_batch_run(tdvh_batch *b, int use_graph, void *stream);
-/* Synchronises `stream` and reports a stalled work queue (a dependency wait
- * that hit its spin limit) as TDVH_ERR_DEVICE. */
+/* Synchronises `stream` (the batch's results are then complete). */
 int tdvh_batch_wait(tdvh_batch *b, void *stream);
 /* Total W*H*D cells of the batch. */
 int64_t tdvh_batch_cells(const tdvh_batch *b);

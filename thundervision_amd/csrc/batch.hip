@@ -1,19 +1,17 @@
 // Batched execution: many stereo pairs of uneven size and disparity range
-// in one persistent launch fed by a device work queue.
+// run as one hipGraph.
 //
-// Work items (one wave each), queued in this order:
-//   phase 0  BT term rows of a pair              (no dependency)
-//   phase 1  4 lines of one SGM direction        (needs all phase-0 items of its pair)
-//            ordered longest line first across all pairs, so the longest
-//            recurrences start first and the launch has a short tail
-//   phase 2  a chunk of pixels: plane sum + WTA  (needs all phase-1 items of its pair)
-// A wave takes the next item with one atomic add on the queue head. Items
-// are taken in queue order, so an item only ever waits for items already
-// taken by running waves: the waits always end. Each wait is still bounded
-// (a spin limit sets an error flag that tdvh_batch_wait reports).
-// Completion counters are published with a device-scope release fence and
-// read with acquire loads. The arithmetic is the per-pair path's (the same
-// device functions), so every pair's disparities equal the plain call's.
+// The batch is planned once into device work-item lists (one wave per item):
+//   BT term rows of every pair                     (one launch for all pairs)
+//   4-line groups of one SGM direction of a pair   (one launch per depth class)
+//   chunks of pixels: plane sum + WTA              (one launch per depth class)
+// Pairs are grouped by m = ceil(D/32), the register shape of the SGM code, so
+// each class's kernel is compiled for its own m and keeps its occupancy; the
+// line items of a class are ordered longest line first across all its pairs,
+// so the longest recurrences start first. Kernel boundaries order the phases;
+// the depth classes run as parallel branches of the graph. The arithmetic is
+// the per-pair path's (the same device functions), so every pair's
+// disparities equal the plain call's.
 #include <algorithm>
 #include <vector>
 
@@ -26,97 +24,53 @@ struct BPair {
     const uint8_t *L, *R;
     int pitch, m;
     int16_t *disp;
-    int need[3]; // items per phase
 };
 
 struct BItem {
     int pair, kind, a, b;
 };
 
-constexpr int BCTL_HEADER = 16; // head, error, spare
-constexpr int SUM_CHUNK = 64;   // pixels per phase-2 item
-constexpr uint32_t SPIN_LIMIT = 1u << 26;
+constexpr int SUM_CHUNK = 64; // pixels per sum item
 
-__device__ __forceinline__ uint32_t load_acquire(const uint32_t *p)
+__global__ __launch_bounds__(256) void batch_bt_kernel(const BPair *__restrict__ pairs,
+                                                       const BItem *__restrict__ items, int n)
 {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (w >= n) return;
+    const BItem it = items[w];
+    const BPair &P = pairs[it.pair];
+    const int Wr = P.a.Wr;
+    for (int y = it.a; y < it.a + it.b; y++)
+        for (int i = lane; i < Wr; i += 64)
+            bt_planes_at(P.L, P.R, P.a.W, P.a.H, P.pitch, P.m, (uint32_t *)P.a.LT,
+                         (uint32_t *)P.a.RT, Wr, i, y);
 }
 
 template <int M, int PF>
-__device__ __forceinline__ void run_item_m(const BPair &P, const BItem &it, int prio)
+__global__ __launch_bounds__(256) void batch_lines_kernel(const BPair *__restrict__ pairs,
+                                                          const BItem *__restrict__ items, int n,
+                                                          int prio)
 {
-    if (it.kind == 1) {
-        sgm_line_group<M, true, true, PF>(P.a, it.a, it.b, it.a, prio);
-    } else {
-        const int j = threadIdx.x & 15, row = (threadIdx.x & 63) >> 4;
-        const long npix = (long)P.a.W * P.a.H;
-        for (int q = row; q < it.b; q += 4) {
-            const long p = (long)it.a + q;
-            if (p < npix) sgm_sum_pixel<M, true, true>(P.a, 8, nullptr, P.disp, P.a.W, p);
-        }
-        (void)j;
-    }
+    const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (w >= n) return;
+    const BItem it = items[w];
+    sgm_line_group<M, true, true, PF>(pairs[it.pair].a, it.a, it.b, it.a, prio);
 }
 
-template <int PF>
-__global__ __launch_bounds__(256) void batch_kernel(const BPair *__restrict__ pairs,
-                                                    const BItem *__restrict__ items, int nitems,
-                                                    int npairs, uint32_t *ctl, int prio)
+template <int M>
+__global__ __launch_bounds__(256) void batch_sum_kernel(const BPair *__restrict__ pairs,
+                                                        const BItem *__restrict__ items, int n)
 {
-    const int lane = threadIdx.x & 63;
-    uint32_t *head = ctl, *err = ctl + 1, *done = ctl + BCTL_HEADER;
-    for (;;) {
-        uint32_t idx = 0;
-        if (lane == 0) idx = atomicAdd(head, 1u);
-        idx = __builtin_amdgcn_readfirstlane(idx);
-        if (idx >= (uint32_t)nitems) break;
-        const BItem it = items[idx];
-        const BPair &P = pairs[it.pair];
-        if (it.kind > 0) {
-            const uint32_t *dep = done + (it.kind - 1) * npairs + it.pair;
-            const uint32_t need = (uint32_t)P.need[it.kind - 1];
-            uint32_t spins = 0;
-            bool ok = true;
-            while (__builtin_amdgcn_readfirstlane(load_acquire(dep)) < need) {
-                if (++spins > SPIN_LIMIT) {
-                    ok = false;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(4);
-            }
-            if (!ok) {
-                if (lane == 0) atomicOr(err, 1u);
-                return;
-            }
-        }
-        if (it.kind == 0) {
-            const int Wr = P.a.Wr;
-            for (int y = it.a; y < it.a + it.b; y++)
-                for (int i = lane; i < Wr; i += 64)
-                    bt_planes_at(P.L, P.R, P.a.W, P.a.H, P.pitch, P.m, (uint32_t *)P.a.LT,
-                                 (uint32_t *)P.a.RT, Wr, i, y);
-        } else {
-            switch (P.m) {
-            case 1: run_item_m<1, PF>(P, it, prio); break;
-            case 2: run_item_m<2, PF>(P, it, prio); break;
-            case 3: run_item_m<3, PF>(P, it, prio); break;
-            case 4: run_item_m<4, PF>(P, it, prio); break;
-            case 5: run_item_m<5, PF>(P, it, prio); break;
-            case 6: run_item_m<6, PF>(P, it, prio); break;
-            case 7: run_item_m<7, PF>(P, it, prio); break;
-            default: run_item_m<8, PF>(P, it, prio); break;
-            }
-        }
-        __threadfence(); // publish this item's results (device-scope release)
-        if (lane == 0) atomicAdd(done + it.kind * npairs + it.pair, 1u);
-        __builtin_amdgcn_s_setprio(0);
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= n) return;
+    const BItem it = items[w];
+    const BPair &P = pairs[it.pair];
+    const int row = (threadIdx.x & 63) >> 4;
+    const long npix = (long)P.a.W * P.a.H;
+    for (int q = row; q < it.b; q += 4) {
+        const long p = (long)it.a + q;
+        if (p < npix) sgm_sum_pixel<M, true, true>(P.a, 8, nullptr, P.disp, P.a.W, p);
     }
-}
-
-__global__ void batch_reset_kernel(uint32_t *ctl, int n)
-{
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        ctl[i] = 0;
 }
 
 } // namespace tdvh
@@ -126,14 +80,14 @@ using namespace tdvh;
 struct tdvh_batch {
     tdvh_ctx *ctx = nullptr;
     int n = 0;
-    std::vector<tdvh_pair> host_pairs;
     tdvh_sgm_params params{};
-    void *mem = nullptr;     // all device memory of the batch
+    void *mem = nullptr; // all device memory of the batch
     size_t mem_bytes = 0;
     BPair *d_pairs = nullptr;
     BItem *d_items = nullptr;
-    uint32_t *d_ctl = nullptr;
-    int nitems = 0, nctl = 0;
+    // item list ranges: BT rows, then per depth class m (1..8) lines and sums
+    int bt_begin = 0, bt_count = 0;
+    int line_begin[9] = {}, line_count[9] = {}, sum_begin[9] = {}, sum_count[9] = {};
     int64_t cells = 0;
     // knobs
     int waves_per_group = 4, prefetch = 4, prio = 1;
@@ -141,6 +95,8 @@ struct tdvh_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t graph_stream = nullptr;
+    hipStream_t branch[9] = {};
+    hipEvent_t fork = nullptr, join[9] = {};
 };
 
 static void drop_graph(tdvh_batch *b)
@@ -182,57 +138,54 @@ extern "C" int tdvh_batch_create(tdvh_ctx *ctx, const tdvh_pair *pairs, int n,
     }
     const size_t trash_off = total;
     total = align(total + 64 * 64);
-    // items
+    // items: BT rows of all pairs, then per depth class the line groups
+    // (longest line first) and the pixel chunks
     std::vector<BItem> items;
-    std::vector<std::pair<int, BItem>> lines; // (length, item)
-    std::vector<int> need0(n, 0), need1(n, 0), need2(n, 0);
-    for (int i = 0; i < n; i++) {
-        const int W = pairs[i].width, H = pairs[i].height;
-        const int rows_per_item = 4;
-        for (int y = 0; y < H; y += rows_per_item) {
-            items.push_back({i, 0, y, std::min(rows_per_item, H - y)});
-            need0[i]++;
-        }
-        for (int dir = 0; dir < 8; dir++) {
-            const int nl = dir_lines(dir, W, H);
-            for (int l0 = 0; l0 < nl; l0 += 4) {
-                // longest line of the group
-                int longest = 0;
-                for (int l = l0; l < std::min(l0 + 4, nl); l++) {
-                    int sx, sy, len;
-                    line_geom(dir, l, W, H, sx, sy, len);
-                    longest = std::max(longest, len);
+    const int rows_per_item = 4;
+    for (int i = 0; i < n; i++)
+        for (int y = 0; y < pairs[i].height; y += rows_per_item)
+            items.push_back({i, 0, y, std::min(rows_per_item, pairs[i].height - y)});
+    tdvh_batch plan;
+    plan.bt_begin = 0;
+    plan.bt_count = (int)items.size();
+    for (int mc = 1; mc <= 8; mc++) {
+        std::vector<std::pair<int, BItem>> lines;
+        for (int i = 0; i < n; i++) {
+            if (split_m(pairs[i].depth) != mc) continue;
+            const int W = pairs[i].width, H = pairs[i].height;
+            for (int dir = 0; dir < 8; dir++) {
+                const int nl = dir_lines(dir, W, H);
+                for (int l0 = 0; l0 < nl; l0 += 4) {
+                    int longest = 0;
+                    for (int l = l0; l < std::min(l0 + 4, nl); l++) {
+                        int sx, sy, len;
+                        line_geom(dir, l, W, H, sx, sy, len);
+                        longest = std::max(longest, len);
+                    }
+                    lines.push_back({longest, BItem{i, 1, dir, l0}});
                 }
-                lines.push_back({longest, BItem{i, 1, dir, l0}});
-                need1[i]++;
             }
         }
-    }
-    std::stable_sort(lines.begin(), lines.end(),
-                     [](const std::pair<int, BItem> &x, const std::pair<int, BItem> &y) {
-                         return x.first > y.first;
-                     });
-    for (auto &l : lines) items.push_back(l.second);
-    // sums: pairs with the shortest longest-line first (their lines end first)
-    std::vector<int> order(n);
-    for (int i = 0; i < n; i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-        return std::max(pairs[x].width, pairs[x].height) < std::max(pairs[y].width, pairs[y].height);
-    });
-    for (int i : order) {
-        const long npix = (long)pairs[i].width * pairs[i].height;
-        for (long p = 0; p < npix; p += SUM_CHUNK) {
-            items.push_back({i, 2, (int)p, (int)std::min<long>(SUM_CHUNK, npix - p)});
-            need2[i]++;
+        std::stable_sort(lines.begin(), lines.end(),
+                         [](const std::pair<int, BItem> &x, const std::pair<int, BItem> &y) {
+                             return x.first > y.first;
+                         });
+        plan.line_begin[mc] = (int)items.size();
+        for (auto &l : lines) items.push_back(l.second);
+        plan.line_count[mc] = (int)items.size() - plan.line_begin[mc];
+        plan.sum_begin[mc] = (int)items.size();
+        for (int i = 0; i < n; i++) {
+            if (split_m(pairs[i].depth) != mc) continue;
+            const long npix = (long)pairs[i].width * pairs[i].height;
+            for (long p = 0; p < npix; p += SUM_CHUNK)
+                items.push_back({i, 2, (int)p, (int)std::min<long>(SUM_CHUNK, npix - p)});
         }
+        plan.sum_count[mc] = (int)items.size() - plan.sum_begin[mc];
     }
     const size_t pairs_off = total;
     total = align(total + sizeof(BPair) * n);
     const size_t items_off = total;
     total = align(total + sizeof(BItem) * items.size());
-    const size_t ctl_off = total;
-    const int nctl = BCTL_HEADER + 3 * n;
-    total = align(total + 4 * (size_t)nctl);
 
     tdvh_batch *b = new tdvh_batch();
     hipError_t e = hipMalloc(&b->mem, total);
@@ -265,20 +218,21 @@ extern "C" int tdvh_batch_create(tdvh_ctx *ctx, const tdvh_pair *pairs, int n,
         q.pitch = p.pitch;
         q.m = m;
         q.disp = p.disp;
-        q.need[0] = need0[i];
-        q.need[1] = need1[i];
-        q.need[2] = need2[i];
     }
     b->ctx = ctx;
     b->n = n;
-    b->host_pairs.assign(pairs, pairs + n);
     b->params = *params;
     b->mem_bytes = total;
     b->d_pairs = (BPair *)(base + pairs_off);
     b->d_items = (BItem *)(base + items_off);
-    b->d_ctl = (uint32_t *)(base + ctl_off);
-    b->nitems = (int)items.size();
-    b->nctl = nctl;
+    b->bt_begin = plan.bt_begin;
+    b->bt_count = plan.bt_count;
+    for (int mc = 0; mc <= 8; mc++) {
+        b->line_begin[mc] = plan.line_begin[mc];
+        b->line_count[mc] = plan.line_count[mc];
+        b->sum_begin[mc] = plan.sum_begin[mc];
+        b->sum_count[mc] = plan.sum_count[mc];
+    }
     b->cells = cells;
     if (hipMemcpy(b->d_pairs, bp.data(), sizeof(BPair) * n, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(b->d_items, items.data(), sizeof(BItem) * items.size(),
@@ -299,6 +253,12 @@ extern "C" int tdvh_batch_destroy(tdvh_batch *b)
     TDVH_TRY(use_device(b->ctx));
     TDVH_CHECK_HIP(hipDeviceSynchronize());
     drop_graph(b);
+    if (b->graph_stream) (void)hipStreamDestroy(b->graph_stream);
+    for (int mc = 0; mc <= 8; mc++) {
+        if (b->branch[mc]) (void)hipStreamDestroy(b->branch[mc]);
+        if (b->join[mc]) (void)hipEventDestroy(b->join[mc]);
+    }
+    if (b->fork) (void)hipEventDestroy(b->fork);
     (void)hipFree(b->mem);
     delete b;
     return TDVH_OK;
@@ -322,27 +282,73 @@ extern "C" int tdvh_batch_config(tdvh_batch *b, int lines_per_wave, int waves_pe
     return TDVH_OK;
 }
 
-static int enqueue_batch(tdvh_batch *b, hipStream_t s)
+template <int M>
+static void launch_class(tdvh_batch *b, hipStream_t s)
 {
-    batch_reset_kernel<<<1, 256, 0, s>>>(b->d_ctl, b->nctl);
-    TDVH_TRY(check_launch("batch_reset_kernel", s));
-    const int threads = 64 * b->waves_per_group;
-    int per_cu = 0;
-    hipError_t e;
-    if (b->prefetch == 8)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<8>, threads, 0);
-    else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<4>, threads, 0);
-    if (e != hipSuccess || per_cu <= 0) per_cu = 1;
-    const int blocks = per_cu * b->ctx->num_cus;
+    const int threads = 64 * b->waves_per_group, wpb = b->waves_per_group;
+    const int nl = b->line_count[M], ns = b->sum_count[M];
+    const BItem *li = b->d_items + b->line_begin[M], *si = b->d_items + b->sum_begin[M];
+    if (nl) {
+        prof_begin(s);
+        if (b->prefetch == 8)
+            batch_lines_kernel<M, 8><<<(nl + wpb - 1) / wpb, threads, 0, s>>>(b->d_pairs, li, nl,
+                                                                            b->prio);
+        else
+            batch_lines_kernel<M, 4><<<(nl + wpb - 1) / wpb, threads, 0, s>>>(b->d_pairs, li, nl,
+                                                                            b->prio);
+        (void)check_launch("batch_lines_kernel", s);
+    }
+    if (ns) {
+        prof_begin(s);
+        batch_sum_kernel<M><<<(ns + 3) / 4, 256, 0, s>>>(b->d_pairs, si, ns);
+        (void)check_launch("batch_sum_kernel", s);
+    }
+}
+
+static int launch_class_m(tdvh_batch *b, int m, hipStream_t s)
+{
+    switch (m) {
+    case 1: launch_class<1>(b, s); break;
+    case 2: launch_class<2>(b, s); break;
+    case 3: launch_class<3>(b, s); break;
+    case 4: launch_class<4>(b, s); break;
+    case 5: launch_class<5>(b, s); break;
+    case 6: launch_class<6>(b, s); break;
+    case 7: launch_class<7>(b, s); break;
+    default: launch_class<8>(b, s); break;
+    }
+    return check_launch("batch_class", s);
+}
+
+// Enqueues the batch on `s`. With fork != 0 (graph capture) the depth
+// classes run on branch streams forked from and joined back into `s`.
+static int enqueue_batch(tdvh_batch *b, hipStream_t s, bool fork)
+{
+    // the lines kernels of a class use lane blocks of 4 lines per wave with
+    // waves_per_group waves per workgroup (the sum kernels keep 4)
     prof_begin(s);
-    if (b->prefetch == 8)
-        batch_kernel<8><<<blocks, threads, 0, s>>>(b->d_pairs, b->d_items, b->nitems, b->n,
-                                                   b->d_ctl, b->prio);
-    else
-        batch_kernel<4><<<blocks, threads, 0, s>>>(b->d_pairs, b->d_items, b->nitems, b->n,
-                                                   b->d_ctl, b->prio);
-    return check_launch("batch_kernel", s);
+    batch_bt_kernel<<<(b->bt_count + 3) / 4, 256, 0, s>>>(b->d_pairs, b->d_items + b->bt_begin,
+                                                          b->bt_count);
+    TDVH_TRY(check_launch("batch_bt_kernel", s));
+    if (!fork) {
+        for (int mc = 1; mc <= 8; mc++)
+            if (b->line_count[mc] || b->sum_count[mc]) TDVH_TRY(launch_class_m(b, mc, s));
+        return TDVH_OK;
+    }
+    if (!b->fork) TDVH_CHECK_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    TDVH_CHECK_HIP(hipEventRecord(b->fork, s));
+    for (int mc = 1; mc <= 8; mc++) {
+        if (!b->line_count[mc] && !b->sum_count[mc]) continue;
+        if (!b->branch[mc])
+            TDVH_CHECK_HIP(hipStreamCreateWithFlags(&b->branch[mc], hipStreamNonBlocking));
+        if (!b->join[mc])
+            TDVH_CHECK_HIP(hipEventCreateWithFlags(&b->join[mc], hipEventDisableTiming));
+        TDVH_CHECK_HIP(hipStreamWaitEvent(b->branch[mc], b->fork, 0));
+        TDVH_TRY(launch_class_m(b, mc, b->branch[mc]));
+        TDVH_CHECK_HIP(hipEventRecord(b->join[mc], b->branch[mc]));
+        TDVH_CHECK_HIP(hipStreamWaitEvent(s, b->join[mc], 0));
+    }
+    return TDVH_OK;
 }
 
 extern "C" int tdvh_batch_run(tdvh_batch *b, int use_graph, void *stream)
@@ -351,12 +357,17 @@ extern "C" int tdvh_batch_run(tdvh_batch *b, int use_graph, void *stream)
     TDVH_REQUIRE(b, "null batch");
     TDVH_TRY(use_device(b->ctx));
     hipStream_t s = (hipStream_t)stream;
-    if (!use_graph) return enqueue_batch(b, s);
+    if (!use_graph) return enqueue_batch(b, s, false);
     if (!b->exec) {
-        TDVH_CHECK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_batch(b, s);
+        // capture on a private stream (the caller's may be the legacy null
+        // stream, which cannot be captured); the graph then launches on `s`
+        if (!b->graph_stream)
+            TDVH_CHECK_HIP(hipStreamCreateWithFlags(&b->graph_stream, hipStreamNonBlocking));
+        hipStream_t cs = b->graph_stream;
+        TDVH_CHECK_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_batch(b, cs, true);
         hipGraph_t g = nullptr;
-        const hipError_t ec = hipStreamEndCapture(s, &g);
+        const hipError_t ec = hipStreamEndCapture(cs, &g);
         TDVH_TRY(rc);
         TDVH_CHECK_HIP(ec);
         b->graph = g;
@@ -372,12 +383,6 @@ extern "C" int tdvh_batch_wait(tdvh_batch *b, void *stream)
     TDVH_REQUIRE(b, "null batch");
     TDVH_TRY(use_device(b->ctx));
     TDVH_CHECK_HIP(hipStreamSynchronize((hipStream_t)stream));
-    uint32_t ctl[2] = {0, 0};
-    TDVH_CHECK_HIP(hipMemcpy(ctl, b->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost));
-    if (ctl[1]) {
-        set_error("batch work queue stalled (a dependency wait hit its spin limit)");
-        return TDVH_ERR_DEVICE;
-    }
     return TDVH_OK;
 }
 
