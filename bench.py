@@ -86,6 +86,52 @@ def bench_batched(tv, synth, torch, ctx, reps):
             "bit_exact_vs_plain": exact}
 
 
+def bench_served(torch, duration=1.0, rate=600.0, frames_per_stream=4, slots=8):
+    """16 streams (4 each at 640x480/D64, 1280x720/D128, 1280x720/D192,
+    1920x1080/D256), Poisson arrivals at `rate` pairs/s in total, frames of
+    stream s from the generator with seed 2000+s and drifting rectangles.
+    Continuous batching over `slots` concurrent slots under a 64 GB budget,
+    against one slot (the plain per-pair call) at the same load."""
+    from thundervision_amd import serve, synth
+    dev = torch.device("cuda:0")
+    shapes = serve.SERVED_STREAMS
+    inputs = {}
+    for s, (W, H, D) in enumerate(shapes):
+        for k in range(frames_per_stream):
+            L, R, _ = synth.pair(W, H, D - 1, 12, 2000 + s, frame=k)
+            inputs[(s, k)] = (torch.from_numpy(L).to(dev), torch.from_numpy(R).to(dev))
+    sched = serve.poisson_schedule(len(shapes), rate, duration)
+    out = {"workload": "16 streams: 4x 640x480/D64, 4x 1280x720/D128, 4x 1280x720/D192, "
+                       "4x 1920x1080/D256; Poisson %.0f pairs/s for %.1f s; BT+SGM8+WTA"
+                       % (rate, duration),
+           "frames": len(sched)}
+    for name, nslots in (("served", slots), ("plain_per_pair", 1)):
+        frames = []
+        for t, s, k in sched:
+            W, H, D = shapes[s]
+            L, R = inputs[(s, k % frames_per_stream)]
+            disp = torch.empty((H, W), dtype=torch.int16, device=dev)
+            frames.append(serve.Frame(t, s, k, W, H, D, (L, R, disp)))
+        ex = serve.GpuExecutor(nslots)
+        sch = serve.Scheduler(ex, nslots, 64 << 30)
+        # warm the slots' scratch with the largest frame shape
+        for slot in range(nslots):
+            f = max(frames, key=lambda f: f.cells)
+            ex.launch(f, slot)
+        torch.cuda.synchronize()
+        ex.start()
+        done = sch.run(frames)
+        torch.cuda.synchronize()
+        st = serve.latency_stats(done)
+        out[name] = {k: round(v, 3) for k, v in st.items()}
+        out[name]["slots"] = nslots
+        out[name]["max_concurrent"] = sch.max_concurrent
+        out[name]["peak_inflight_GB"] = round(sch.peak_in_use / 2**30, 2)
+    out["offered_MDE/s"] = round(sum(W * H * D for t, s, k in sched
+                                     for (W, H, D) in [shapes[s]]) / duration / 1e6, 1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +140,7 @@ def main():
     ap.add_argument("--mode", choices=["fused", "staged"], default="fused")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--profile-run", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0)
     args = ap.parse_args()
@@ -171,6 +218,10 @@ def main():
     if not args.no_batched:
         batched = bench_batched(tv, synth, torch, ctx, max(3, args.steps // 5))
 
+    served = None
+    if not args.no_served:
+        served = bench_served(torch)
+
     cpu = None
     if not args.no_cpu_baseline:
         import oracle
@@ -215,6 +266,7 @@ def main():
                        "ms_per_step": round(sec_other * 1e3, 4)},
         "kernels_ms": {k: round(v[0] / v[1], 4) for k, v in table.items()},
         "batched": batched,
+        "served": served,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -1,0 +1,103 @@
+"""Serving loop: admission/budget logic on the CPU with a fake executor, and
+on the GPU: every frame's disparities are the same alone and in the mix."""
+import numpy as np
+import pytest
+
+from thundervision_amd import serve
+
+
+class FakeExecutor:
+    """Deterministic clock; a frame takes cells / rate seconds."""
+
+    def __init__(self, rate=1e9):
+        self.t = 0.0
+        self.rate = rate
+        self.running = {}
+        self.launched = []
+
+    def now(self):
+        self.t += 1e-4
+        return self.t
+
+    def launch(self, f, slot):
+        self.running[slot] = self.t + f.cells / self.rate
+        self.launched.append((f.stream, f.index, slot))
+
+    def poll(self, slot):
+        if slot in self.running and self.running[slot] <= self.t:
+            return self.running.pop(slot)
+        return None
+
+
+def frames_from(schedule, shapes):
+    return [serve.Frame(t, s, k, *shapes[s]) for t, s, k in schedule]
+
+
+def test_schedule_is_poisson_like_and_seeded():
+    a = serve.poisson_schedule(16, 600.0, 2.0)
+    b = serve.poisson_schedule(16, 600.0, 2.0)
+    assert a == b
+    assert 1000 < len(a) < 1400  # ~1200 expected
+    assert all(x[0] <= y[0] for x, y in zip(a, a[1:]))
+
+
+def test_budget_and_slots_are_respected():
+    sched = serve.poisson_schedule(16, 600.0, 0.5)
+    frames = frames_from(sched, serve.SERVED_STREAMS)
+    big = serve.frame_footprint(1920, 1080, 256)
+    ex = FakeExecutor(rate=2e11)
+    sch = serve.Scheduler(ex, slots=4, budget_bytes=int(2.5 * big))
+    done = sch.run(frames)
+    assert len(done) == len(frames)
+    assert sch.peak_in_use <= int(2.5 * big)
+    assert sch.max_concurrent <= 4
+    assert all(f.t_done >= f.t_arrival for f in done)
+    # frames of one stream complete in arrival order under FIFO admission
+    for s in range(16):
+        idx = [f.index for f in sorted(done, key=lambda f: f.t_start) if f.stream == s]
+        assert idx == sorted(idx)
+
+
+def test_oversized_frame_is_rejected():
+    ex = FakeExecutor()
+    sch = serve.Scheduler(ex, slots=2, budget_bytes=1000)
+    with pytest.raises(ValueError, match="budget"):
+        sch.run([serve.Frame(0.0, 0, 0, 640, 480, 64)])
+
+
+def test_latency_stats():
+    fs = []
+    for k in range(100):
+        f = serve.Frame(k * 0.01, 0, k, 10, 10, 10)
+        f.t_done = f.t_arrival + (k + 1) * 1e-3
+        fs.append(f)
+    st = serve.latency_stats(fs)
+    assert abs(st["p50_ms"] - 51) < 1.01 and abs(st["p99_ms"] - 99) < 1.01
+
+
+@pytest.mark.gpu
+def test_frames_identical_alone_and_in_the_mix(tv, dev):
+    import torch
+    from thundervision_amd import synth
+    shapes = [(160, 120, 32), (320, 240, 64), (200, 150, 80), (640, 480, 128)]
+    streams = [shapes[i % 4] for i in range(8)]
+    sched = serve.poisson_schedule(8, 2000.0, 0.05, seed_base=3000)
+    frames = []
+    for t, s, k in sched:
+        W, H, D = streams[s]
+        L, R, _ = synth.pair(W, H, D - 1, 8, 3000 + s, frame=k)
+        Ld, Rd = torch.from_numpy(L).to(dev), torch.from_numpy(R).to(dev)
+        disp = torch.full((H, W), -1, dtype=torch.int16, device=dev)
+        frames.append(serve.Frame(t, s, k, W, H, D, (Ld, Rd, disp)))
+    ex = serve.GpuExecutor(slots=4)
+    sch = serve.Scheduler(ex, slots=4, budget_bytes=8 << 30)
+    ex.start()
+    done = sch.run(frames)
+    torch.cuda.synchronize()
+    assert len(done) == len(frames) and sch.max_concurrent >= 2
+    for f in frames:
+        Ld, Rd, disp = f.payload
+        alone = tv.stereo_bt_sgm(Ld, Rd, f.depth)
+        assert torch.equal(alone, disp), (f.stream, f.index)
+    st = serve.latency_stats(done)
+    assert st["p99_ms"] >= st["p50_ms"] > 0
