@@ -30,6 +30,28 @@ def b_alg(W, H, D):
     return 8 * W * H * D + 4 * W * H
 
 
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed PMC summary
+    (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_sgm_fused_720p_d128.txt")
+    try:
+        rows, scale = {}, {}
+        for line in open(path):
+            if line.startswith("#") or not line.strip():
+                continue
+            parts = line.split()
+            if parts[0].startswith("scale_"):
+                scale[parts[0]] = float(parts[1])
+            else:
+                rows[parts[0]] = (float(parts[1]), float(parts[2]))
+        for name, (f, w) in rows.items():
+            if kernel in name:
+                return int((f * scale["scale_fetch"] + w * scale["scale_write"]) * 1024)
+    except (OSError, KeyError, ValueError, IndexError):
+        pass
+    return None
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -257,7 +279,8 @@ def main():
                    "path": args.mode, "width": W, "height": H, "depth": D},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": dom,
+                     "traffic": measured_traffic(dom) if args.mode == "fused" else None,
+                     "kernel": dom,
                      "kernel_avg_ms": round(dom_ms, 4),
                      "alg_bytes_per_launch": b_alg(W, H, D)},
         "pipeline_frac_of_hbm_roofline": round(b_alg(W, H, D) / (sec * 1e9) / HBM_PEAK_GBS, 4),
