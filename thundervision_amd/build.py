@@ -2,7 +2,7 @@
 
 libtdvh.so     HIP kernels + C ABI (include/tdvh.h), gfx950, hipcc
 libtdvsynth.so seeded synthetic stereo-pair generator (plain C, gcc)
-libtdvision.so C++ stage layer (include/tdvision/*.hpp) over libtdvh.so
+libtdvision_mi355x.so C++ stage layer (include/tdvision/*.hpp) over libtdvh.so
 _bin/test_stages  C++ tests of the stage layer (run by tests/test_stages_cpp.py)
 """
 import os
@@ -57,7 +57,7 @@ def build(verbose=False):
                   for f in ("runtime.cpp", "images.cpp", "stages.cpp", "matchers.cpp")]
     cxx = ["g++", "-O2", "-std=c++17", "-fPIC", "-D__HIP_PLATFORM_AMD__",
            "-I" + INCLUDE, "-I/opt/rocm/include"]
-    stlib = os.path.join(PKG, "libtdvision.so")
+    stlib = os.path.join(PKG, "libtdvision_mi355x.so")
     _run(cxx + ["-shared", "-o", stlib + ".tmp"] + stage_srcs +
          ["-L" + PKG, "-ltdvh", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN",
           "-Wl,-rpath,/opt/rocm/lib", "-pthread"])
@@ -65,7 +65,7 @@ def build(verbose=False):
     os.makedirs(os.path.join(PKG, "_bin"), exist_ok=True)
     tbin = os.path.join(PKG, "_bin", "test_stages")
     _run(cxx + ["-o", tbin + ".tmp", os.path.join(ROOT, "tests", "cpp", "test_stages.cpp"),
-                "-L" + PKG, "-ltdvision", "-ltdvh", "-L/opt/rocm/lib", "-lamdhip64",
+                "-L" + PKG, "-ltdvision_mi355x", "-ltdvh", "-L/opt/rocm/lib", "-lamdhip64",
                 "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath,/opt/rocm/lib", "-pthread"])
     os.replace(tbin + ".tmp", tbin)
     synth = os.path.join(PKG, "libtdvsynth.so")

@@ -82,6 +82,7 @@ struct tdvh_ctx {
     struct Slot {
         void *ptr = nullptr;
         size_t bytes = 0;
+        uint64_t tag = 0; // what the contents were built for (0: nothing)
     };
     std::unordered_map<std::string, Slot> slots;
 
@@ -99,6 +100,8 @@ struct tdvh_ctx {
 
     // Returns a device buffer of at least `bytes` for `name` (reused).
     int scratch(const char *name, size_t bytes, void **out);
+    // The slot itself (after scratch()), to read or set its tag.
+    Slot &slot(const char *name) { return slots[name]; }
     int release_all();
     size_t held() const;
 };

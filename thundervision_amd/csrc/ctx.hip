@@ -94,6 +94,7 @@ int tdvh_ctx::scratch(const char *name, size_t bytes, void **out)
             s.ptr = nullptr;
             s.bytes = 0;
         }
+        s.tag = 0;
         hipError_t e = hipMalloc(&s.ptr, bytes);
         if (e != hipSuccess) {
             (void)hipGetLastError();

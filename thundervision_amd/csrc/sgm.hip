@@ -16,7 +16,10 @@
 // direction's lines are independent, so all 8 directions run in one launch
 // and each writes its own path-cost plane; a second kernel sums the planes
 // and takes the lowest-index argmin (wtacpu.cpp:20-33).
+#include <algorithm>
 #include <cstdlib>
+#include <string>
+#include <vector>
 
 #include "sgm_device.hpp"
 
@@ -35,9 +38,28 @@ static int lines_prefetch()
     static int pf = [] {
         const char *e = getenv("TDVH_SGM_PREFETCH");
         const int v = e ? atoi(e) : 4;
-        return (v == 8 || v == 16) ? v : 4;
+        return (v == 8 || v == 12 || v == 16) ? v : 4;
     }();
     return pf;
+}
+
+// Buffer-addressed line kernel (byte planes below 2^31 bytes) unless
+// TDVH_SGM_LINES=flat selects the flat-address kernel.
+static bool lines_buf_enabled()
+{
+    static const bool on = [] {
+        const char *e = getenv("TDVH_SGM_LINES");
+        return !(e && std::string(e) == "flat");
+    }();
+    return on;
+}
+
+template <int M, bool FUSED>
+static void launch_lines_buf(const SgmArgs &a, const uint32_t *order, int groups, int prio,
+                             hipStream_t s)
+{
+    if (a.D < 32 * M) sgm_lines_buf_kernel<M, FUSED, true><<<groups, 64, 0, s>>>(a, order, prio);
+    else sgm_lines_buf_kernel<M, FUSED, false><<<groups, 64, 0, s>>>(a, order, prio);
 }
 
 template <int M, bool FUSED, bool EOUT>
@@ -83,10 +105,74 @@ static LineSet make_lineset(int W, int H, int dir_mask)
     return ls;
 }
 
-int launch_sgm_lines(const SgmArgs &a, int dir_mask, bool eout, hipStream_t s)
+// Device table of the line groups (4 consecutive lines of one direction)
+// of every enabled direction, longest first: with one wave per workgroup the
+// dispatcher then runs a longest-processing-time-first list schedule.
+// Entry: plane slot << 28 | direction << 24 | first line. Built and uploaded
+// once per (W, H, dir_mask) and context.
+static int line_order(tdvh_ctx *ctx, int W, int H, int dir_mask, const uint32_t **out,
+                      int *groups)
+{
+    TDVH_REQUIRE(W + H < (1 << 24), "image %dx%d too large for the line table", W, H);
+    std::vector<std::pair<int, uint32_t>> g;
+    int slot = 0;
+    for (int dir = 0; dir < 8; dir++) {
+        if (!(dir_mask & (1 << dir))) continue;
+        const int n = dir_lines(dir, W, H);
+        for (int l0 = 0; l0 < n; l0 += 4) {
+            int kmax = 0;
+            for (int l = l0; l < l0 + 4 && l < n; l++) {
+                int sx, sy, len;
+                line_geom(dir, l, W, H, sx, sy, len);
+                kmax = len > kmax ? len : kmax;
+            }
+            g.push_back({kmax, (uint32_t)slot << 28 | (uint32_t)dir << 24 | (uint32_t)l0});
+        }
+        slot++;
+    }
+    std::stable_sort(g.begin(), g.end(),
+                     [](const std::pair<int, uint32_t> &x, const std::pair<int, uint32_t> &y) {
+                         return x.first > y.first;
+                     });
+    void *buf = nullptr;
+    TDVH_TRY(ctx->scratch("sgm_line_order", g.size() * 4, &buf));
+    tdvh_ctx::Slot &sl = ctx->slot("sgm_line_order");
+    const uint64_t tag = ((uint64_t)W << 32) | ((uint64_t)H << 8) | (uint64_t)dir_mask;
+    if (sl.tag != tag) {
+        std::vector<uint32_t> h(g.size());
+        for (size_t i = 0; i < g.size(); i++) h[i] = g[i].second;
+        // queued launches may still read the previous table
+        TDVH_CHECK_HIP(hipDeviceSynchronize());
+        TDVH_CHECK_HIP(hipMemcpy(buf, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+        sl.tag = tag;
+    }
+    *out = (const uint32_t *)buf;
+    *groups = (int)g.size();
+    return TDVH_OK;
+}
+
+int launch_sgm_lines(tdvh_ctx *ctx, const SgmArgs &a, int dir_mask, bool eout, hipStream_t s)
 {
     const LineSet ls = make_lineset(a.W, a.H, dir_mask);
     if (ls.ndirs == 0) return TDVH_OK;
+    if (eout && lines_buf_enabled() && buf_lines_fit(a)) {
+        const uint32_t *order = nullptr;
+        int groups = 0;
+        TDVH_TRY(line_order(ctx, a.W, a.H, dir_mask, &order, &groups));
+        prof_begin(s);
+        const bool fused = a.C == nullptr;
+        switch (split_m(a.D)) {
+#define CASE(m)                                                                                    \
+    case m:                                                                                        \
+        if (fused) launch_lines_buf<m, true>(a, order, groups, ls.prio, s);                        \
+        else launch_lines_buf<m, false>(a, order, groups, ls.prio, s);                             \
+        break;
+            CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+        default: set_error("depth %d above %d", a.D, MAX_DEPTH); return TDVH_ERR_UNSUPPORTED;
+        }
+        return check_launch("sgm_lines_kernel", s);
+    }
     prof_begin(s);
     switch (split_m(a.D)) {
 #define CASE(m)                                                                                    \
@@ -208,7 +294,7 @@ static int sgm_std(tdvh_ctx *ctx, const uint32_t *Csplit, const uint8_t *left,
             a.C = (const uint32_t *)cb;
         }
     }
-    TDVH_TRY(launch_sgm_lines(a, params->dir_mask, eout, s));
+    TDVH_TRY(launch_sgm_lines(ctx, a, params->dir_mask, eout, s));
     uint32_t *Ssplit = nullptr;
     if (S_out) {
         TDVH_REQUIRE(S_out->width == W && S_out->height == H && S_out->depth == D &&

@@ -253,6 +253,231 @@ __device__ __forceinline__ void sgm_line_group(const SgmArgs &a, int dir, int li
         if (k0 + u < kmax) body(ring[u], k0 + u);
 }
 
+// ---- buffer-addressed line scan ---------------------------------------------
+//
+// The same scan as sgm_line_group, addressed through raw buffer resources:
+// per-lane 32-bit byte offsets advance by a constant per step (no 64-bit
+// address arithmetic), loads past a line's end read harmless data or 0 (the
+// hardware range check makes any offset safe), and the stores of lanes past
+// their line use an offset beyond the plane, which the range check drops.
+// Cells with d > x need no mask: the term planes carry 0xFFFF right terms
+// for x' < 0, which give C >= 0xFE01, a cost above every valid path cost.
+// Needs every plane below 2^31 bytes (buf_lines_fit).
+
+typedef __amdgpu_buffer_rsrc_t buf_t;
+typedef unsigned int tdvh_u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int tdvh_u32x3 __attribute__((ext_vector_type(3)));
+typedef unsigned int tdvh_u32x4 __attribute__((ext_vector_type(4)));
+constexpr uint32_t BUF_OOB = 0x80000000u;
+
+__device__ __forceinline__ buf_t buf_rsrc(const void *base, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes,
+                                             0x00020000);
+}
+
+// N consecutive dwords at byte offset off (0 outside the buffer).
+template <int N, int I = 0>
+__device__ __forceinline__ void buf_load(buf_t r, uint32_t off, uint32_t *dst)
+{
+    if constexpr (N - I >= 4) {
+        const tdvh_u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off + 4 * I, 0, 0);
+        dst[I] = v.x, dst[I + 1] = v.y, dst[I + 2] = v.z, dst[I + 3] = v.w;
+        buf_load<N, I + 4>(r, off, dst);
+    } else if constexpr (N - I == 3) {
+        const tdvh_u32x3 v = __builtin_amdgcn_raw_buffer_load_b96(r, off + 4 * I, 0, 0);
+        dst[I] = v.x, dst[I + 1] = v.y, dst[I + 2] = v.z;
+    } else if constexpr (N - I == 2) {
+        const tdvh_u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, off + 4 * I, 0, 0);
+        dst[I] = v.x, dst[I + 1] = v.y;
+    } else if constexpr (N - I == 1) {
+        dst[I] = __builtin_amdgcn_raw_buffer_load_b32(r, off + 4 * I, 0, 0);
+    }
+}
+
+// N consecutive dwords to byte offset off (dropped outside the buffer).
+template <int N>
+__device__ __forceinline__ void buf_store(buf_t r, uint32_t off, const uint32_t *src)
+{
+    if constexpr (N == 4) {
+        __builtin_amdgcn_raw_buffer_store_b128(tdvh_u32x4{src[0], src[1], src[2], src[3]}, r,
+                                               off, 0, 0);
+    } else if constexpr (N == 3) {
+        __builtin_amdgcn_raw_buffer_store_b96(tdvh_u32x3{src[0], src[1], src[2]}, r, off, 0, 0);
+    } else if constexpr (N == 2) {
+        __builtin_amdgcn_raw_buffer_store_b64(tdvh_u32x2{src[0], src[1]}, r, off, 0, 0);
+    } else if constexpr (N == 1) {
+        __builtin_amdgcn_raw_buffer_store_b32(src[0], r, off, 0, 0);
+    }
+}
+
+template <int M, bool FUSED, bool PAD, int PF>
+__device__ __forceinline__ void sgm_line_group_buf(const SgmArgs &a, int dir, int line0, int s,
+                                                   int prio)
+{
+    constexpr int NW = FUSED ? 3 + 3 * M : M; // prefetched words per step
+    const int lane = threadIdx.x & 63;
+    const int line = line0 + (lane >> 4);
+    const int j = lane & 15;
+    const int W = a.W, H = a.H;
+    int sx = 0, sy = 0, len = 0;
+    if (line < dir_lines(dir, W, H)) line_geom(dir, line, W, H, sx, sy, len);
+    const int dx = dir_dx(dir), dy = dir_dy(dir);
+    const int step = dy * W + dx, p0 = sy * W + sx;
+    const int kmax = max(max(__builtin_amdgcn_readlane(len, 0), __builtin_amdgcn_readlane(len, 16)),
+                         max(__builtin_amdgcn_readlane(len, 32), __builtin_amdgcn_readlane(len, 48)));
+    if (prio) {
+        const int longest = max(W, H);
+        if (kmax * 4 >= longest * 3) __builtin_amdgcn_s_setprio(3);
+        else if (kmax * 2 >= longest) __builtin_amdgcn_s_setprio(2);
+        else if (kmax * 4 >= longest) __builtin_amdgcn_s_setprio(1);
+    }
+
+    const buf_t out = buf_rsrc(a.planes + (size_t)s * a.plane_bytes, (uint32_t)a.plane_bytes);
+    uint32_t oE = (uint32_t)p0 * (32 * M) + j * (2 * M);
+    const uint32_t dE = (uint32_t)(step * (32 * M));
+
+    buf_t src[FUSED ? 6 : 1];
+    uint32_t fa = 0, fb = 0, da = 0, db = 0; // fetch offsets and their steps
+    if constexpr (FUSED) {
+        const uint32_t lt_plane = (uint32_t)W * H, rt_plane = (uint32_t)a.Wr * H;
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            src[t] = buf_rsrc(a.LT + (size_t)t * lt_plane, lt_plane * 4);
+            src[3 + t] = buf_rsrc(a.RT + (size_t)t * rt_plane, rt_plane * 4);
+        }
+        fa = (uint32_t)p0 * 4;
+        da = (uint32_t)(step * 4);
+        fb = (uint32_t)(sy * a.Wr + (W - 1 - sx) + j * M) * 4;
+        db = (uint32_t)((dy * a.Wr - dx) * 4);
+    } else {
+        src[0] = buf_rsrc(a.C, (uint32_t)W * H * (64 * M));
+        fa = (uint32_t)p0 * (64 * M) + j * (4 * M);
+        da = (uint32_t)(step * (64 * M));
+    }
+
+    uint32_t ring[PF][NW];
+    auto fetch = [&](uint32_t(&slot)[NW]) {
+        if constexpr (FUSED) {
+#pragma unroll
+            for (int t = 0; t < 3; t++) slot[t] = __builtin_amdgcn_raw_buffer_load_b32(src[t], fa, 0, 0);
+#pragma unroll
+            for (int t = 0; t < 3; t++) buf_load<M>(src[3 + t], fb, slot + 3 + t * M);
+            fb += db;
+        } else {
+            buf_load<M>(src[0], fa, slot);
+        }
+        fa += da;
+    };
+#pragma unroll
+    for (int u = 0; u < PF; u++) fetch(ring[u]);
+
+    // L' = 0 and m = 0 make the first step return L = C (path start).
+    uint32_t Lp[M];
+#pragma unroll
+    for (int r = 0; r < M; r++) Lp[r] = 0;
+    uint32_t m = 0;
+    const uint32_t p1 = pk_splat(a.P1);
+    uint32_t padm[M];
+#pragma unroll
+    for (int r = 0; r < M; r++) padm[r] = cell_mask<M>(j, r, a.D, 1 << 30);
+
+    // Costs of one step from a ring slot.
+    auto costs = [&](const uint32_t(&slot)[NW], uint32_t(&c)[M]) {
+        if constexpr (FUSED) {
+#pragma unroll
+            for (int r = 0; r < M; r++) {
+                const uint32_t l2 = slot[0], lmn = slot[1], lmx = slot[2];
+                const uint32_t r2 = slot[3 + r], rmn = slot[3 + M + r], rmx = slot[3 + 2 * M + r];
+                const uint32_t aa = pk_max(pk_subs(l2, rmx), pk_subs(rmn, l2));
+                const uint32_t bb = pk_max(pk_subs(r2, lmx), pk_subs(lmn, r2));
+                c[r] = pk_min(aa, bb);
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < M; r++) c[r] = slot[r];
+        }
+        if constexpr (PAD) {
+#pragma unroll
+            for (int r = 0; r < M; r++) c[r] |= padm[r];
+        }
+    };
+
+    // Software pipeline: c holds the costs of step k while slot (u+1) % PF
+    // holds the data of step k+1, fetched PF steps earlier. A body runs the
+    // recurrence of step k and the cost computation of step k+1 (independent
+    // work the scheduler interleaves), then refills that slot. The scheduling
+    // barrier keeps each body's waits on its own slot, so the compiler cannot
+    // hoist later steps' cost math (and their waits) above earlier steps.
+    uint32_t c[M];
+    costs(ring[0], c);
+    fetch(ring[0]);
+    auto body = [&](int u, int k) {
+        uint32_t E[M];
+        sgm_step<M>(Lp, c, m, j, p1, (uint32_t)a.P2, E);
+        const uint32_t so = k < len ? oE : BUF_OOB;
+        oE += dE;
+        if constexpr (M % 2 == 0) {
+            uint32_t w[M / 2];
+#pragma unroll
+            for (int r = 0; r < M; r += 2) w[r / 2] = pack_e(E[r], E[r + 1]);
+            if constexpr (M == 8) {
+                buf_store<4>(out, so, w);
+            } else {
+                buf_store<M / 2>(out, so, w);
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < M; r++)
+                __builtin_amdgcn_raw_buffer_store_b16((unsigned short)pack_e(E[r], 0u), out,
+                                                      so + 2 * r, 0, 0);
+        }
+        costs(ring[(u + 1) % PF], c);
+        // refill only after the slot's words are consumed, so the loads
+        // reuse its registers instead of needing a second copy of the ring
+        __builtin_amdgcn_sched_barrier(0);
+        fetch(ring[(u + 1) % PF]);
+        __builtin_amdgcn_sched_barrier(0);
+    };
+
+    int k0 = 0;
+    for (; k0 + PF <= kmax; k0 += PF) {
+#pragma unroll
+        for (int u = 0; u < PF; u++) body(u, k0 + u);
+    }
+#pragma unroll
+    for (int u = 0; u < PF; u++)
+        if (k0 + u < kmax) body(u, k0 + u);
+}
+
+// Prefetch depth of the buffer line kernel: 8 steps for up to 4 words per
+// lane and plane (<= 248 VGPRs), 4 above; two waves per SIMD. Fewer waves
+// per SIMD shorten the longest lines (the launch's critical path), which
+// otherwise share their SIMD's issue slots with two or three other waves.
+template <int M> constexpr int buf_lines_pf() { return M <= 4 ? 8 : 4; }
+
+// One wave per workgroup, so the dispatcher starts the next line group as
+// soon as any wave ends; `order` lists the groups longest first (entry:
+// plane slot << 28 | direction << 24 | first line).
+template <int M, bool FUSED, bool PAD>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void
+sgm_lines_buf_kernel(const SgmArgs a, const uint32_t *__restrict__ order, int prio)
+{
+    const uint32_t e = order[blockIdx.x];
+    sgm_line_group_buf<M, FUSED, PAD, buf_lines_pf<M>()>(a, (int)((e >> 24) & 0xF),
+                                                         (int)(e & 0xFFFFFF), (int)(e >> 28), prio);
+}
+
+// Whether sgm_line_group_buf can address every plane of a (byte planes).
+__host__ __device__ inline bool buf_lines_fit(const SgmArgs &a)
+{
+    const size_t lim = (size_t)1 << 31;
+    const size_t m = (size_t)((a.D + 31) / 32);
+    if (a.plane_bytes >= lim) return false;
+    if (a.C) return (size_t)a.W * a.H * 64 * m < lim;
+    return (size_t)a.W * a.H * 4 < lim && (size_t)a.Wr * a.H * 4 < lim;
+}
+
 template <int M, bool FUSED, bool EOUT, int PF>
 __global__ __launch_bounds__(256) void sgm_lines_kernel(const SgmArgs a, const LineSet ls)
 {
@@ -274,8 +499,13 @@ __device__ __forceinline__ void bt_planes_at(const uint8_t *__restrict__ L,
 {
     const uint8_t *lr = L + (size_t)y * pitch;
     const uint8_t *rr = R + (size_t)y * pitch;
+    // x < 0 (cells with d > x): all three terms 0xFFFF, so the packed BT
+    // formula gives a cost of at least 0xFE01 there
     auto terms = [&](const uint8_t *row, int x, int &v2, int &mn, int &mx) {
-        x = x < 0 ? 0 : x;
+        if (x < 0) {
+            v2 = mn = mx = 0xFFFF;
+            return;
+        }
         const int c = row[x];
         const int a = c + row[x > 0 ? x - 1 : 0], b = c + row[x < W - 1 ? x + 1 : W - 1];
         v2 = 2 * c;
