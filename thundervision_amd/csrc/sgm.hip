@@ -204,11 +204,33 @@ int launch_bt_planes(const uint8_t *L, const uint8_t *R, int W, int H, int pitch
     return check_launch("bt_planes_kernel", s);
 }
 
+// Wide-lane plane sum (16 bytes per lane and plane) for power-of-two word
+// counts unless TDVH_SGM_SUM=narrow selects one 16-lane row per pixel.
+static bool sum_wide_enabled()
+{
+    static const bool on = [] {
+        const char *e = getenv("TDVH_SGM_SUM");
+        return !(e && std::string(e) == "narrow");
+    }();
+    return on;
+}
+
 template <int M>
 static void launch_sum_m(const SgmArgs &a, int nplanes, bool eout, uint32_t *S, int16_t *disp,
                          int pitch, int blocks, hipStream_t s)
 {
     const bool fused = a.C == nullptr;
+    if constexpr (M == 1 || M == 2 || M == 4 || M == 8) {
+        if (eout && sum_wide_enabled()) {
+            const long threads = (long)a.W * a.H * 2 * M;
+            const int wb = (int)((threads + 255) / 256);
+            if (fused)
+                sgm_sum_wide_kernel<M, true><<<wb, 256, 0, s>>>(a, nplanes, S, disp, pitch);
+            else
+                sgm_sum_wide_kernel<M, false><<<wb, 256, 0, s>>>(a, nplanes, S, disp, pitch);
+            return;
+        }
+    }
     if (eout && fused)
         sgm_sum_wta_kernel<M, true, true><<<blocks, 256, 0, s>>>(a, nplanes, S, disp, pitch);
     else if (eout)
