@@ -234,8 +234,10 @@ typedef struct tdvh_batch tdvh_batch;
 int tdvh_batch_create(tdvh_ctx *ctx, const tdvh_pair *pairs, int n,
                       const tdvh_sgm_params *params, tdvh_batch **out);
 int tdvh_batch_destroy(tdvh_batch *b);
-/* Tile configuration (autotuner knobs): lines per workgroup, waves per
- * workgroup, and prefetch depth; 0 keeps the current value. */
+/* Tile configuration (autotuner knobs): lines per wave (4), waves per
+ * workgroup (1, 2, 4; default 1) and prefetch depth of the SGM line scans
+ * (4 or 8 steps; default 8, at most 4 for D > 128); 0 keeps the current
+ * value. */
 int tdvh_batch_config(tdvh_batch *b, int lines_per_wave, int waves_per_group,
                       int prefetch);
 /* Enqueues the whole batch on `stream`. With use_graph != 0 the launch

@@ -13,6 +13,7 @@
 // the per-pair path's (the same device functions), so every pair's
 // disparities equal the plain call's.
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "sgm_device.hpp"
@@ -57,6 +58,46 @@ __global__ __launch_bounds__(256) void batch_lines_kernel(const BPair *__restric
     sgm_line_group<M, true, true, PF>(pairs[it.pair].a, it.a, it.b, it.a, prio);
 }
 
+// Buffer-addressed line groups of one depth class (all 8 planes per pair);
+// waves_per_group waves per workgroup, two waves per SIMD.
+template <int M, bool PAD, int PF>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void
+batch_lines_buf_kernel(const BPair *__restrict__ pairs, const BItem *__restrict__ items, int n,
+                       int prio)
+{
+    const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) +
+                                                 (threadIdx.x >> 6));
+    if (w >= n) return;
+    const BItem it = items[w];
+    sgm_line_group_buf<M, true, PAD, PF>(pairs[it.pair].a, it.a, it.b, it.a, prio);
+}
+
+// Plane sum + WTA with 16-byte lanes (2M lanes per pixel, 64/2M pixels per
+// wave, one pass), for power-of-two word counts. items: one per pair of the
+// class, {pair, 3, first wave, waves}, in increasing first wave; a wave finds
+// its pair by binary search (wave-uniform, scalar loads).
+template <int M>
+__global__ __launch_bounds__(256) void batch_sum_wide_kernel(const BPair *__restrict__ pairs,
+                                                             const BItem *__restrict__ items,
+                                                             int n, int waves)
+{
+    constexpr int G = 2 * M;
+    const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (w >= waves) return;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].a <= w) lo = mid;
+        else hi = mid - 1;
+    }
+    const BItem it = items[lo];
+    const BPair &P = pairs[it.pair];
+    const int lane = threadIdx.x & 63;
+    const long p = (long)(w - it.a) * (64 / G) + lane / G;
+    if (p < (long)P.a.W * P.a.H)
+        sgm_sum_pixel_wide<M, true>(P.a, 8, nullptr, P.disp, P.a.W, p, lane % G);
+}
+
 template <int M>
 __global__ __launch_bounds__(256) void batch_sum_kernel(const BPair *__restrict__ pairs,
                                                         const BItem *__restrict__ items, int n)
@@ -87,17 +128,39 @@ struct tdvh_batch {
     BItem *d_items = nullptr;
     // item list ranges: BT rows, then per depth class m (1..8) lines and sums
     int bt_begin = 0, bt_count = 0;
-    int line_begin[9] = {}, line_count[9] = {}, sum_begin[9] = {}, sum_count[9] = {};
+    // segments: per depth class, consecutive groups of its pairs; a
+    // segment's plane sums overlap the next segment's line scans
+    struct Seg {
+        int mc;
+        int line_begin, line_count; // line-group items, longest first
+        int sum_begin, sum_count;   // sum items
+        int sum_waves;              // waves of the wide sum kernel
+    };
+    std::vector<Seg> segs;
+    bool class_pad[9] = {}; // some pair of the class has padding cells (D < 32m)
+    bool class_buf[9] = {}; // every pair of the class fits the buffer line kernel
     int64_t cells = 0;
     // knobs
-    int waves_per_group = 4, prefetch = 4, prio = 1;
+    int waves_per_group = 1, prefetch = 8, prio = 1;
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t graph_stream = nullptr;
-    hipStream_t branch[9] = {};
-    hipEvent_t fork = nullptr, join[9] = {};
+    hipStream_t branch[9] = {}, sum_branch[9] = {};
+    hipEvent_t fork = nullptr, join[9] = {}, sum_join[9] = {};
+    std::vector<hipEvent_t> seg_done; // per segment: its line scans finished
 };
+
+// Segments per depth class (TDVH_BATCH_SEGMENTS, default 4).
+static int batch_segments()
+{
+    static const int n = [] {
+        const char *e = getenv("TDVH_BATCH_SEGMENTS");
+        const int v = e ? atoi(e) : 4;
+        return v < 1 ? 1 : (v > 16 ? 16 : v);
+    }();
+    return n;
+}
 
 static void drop_graph(tdvh_batch *b)
 {
@@ -149,38 +212,83 @@ extern "C" int tdvh_batch_create(tdvh_ctx *ctx, const tdvh_pair *pairs, int n,
     plan.bt_begin = 0;
     plan.bt_count = (int)items.size();
     for (int mc = 1; mc <= 8; mc++) {
-        std::vector<std::pair<int, BItem>> lines;
-        for (int i = 0; i < n; i++) {
-            if (split_m(pairs[i].depth) != mc) continue;
-            const int W = pairs[i].width, H = pairs[i].height;
-            for (int dir = 0; dir < 8; dir++) {
-                const int nl = dir_lines(dir, W, H);
-                for (int l0 = 0; l0 < nl; l0 += 4) {
-                    int longest = 0;
-                    for (int l = l0; l < std::min(l0 + 4, nl); l++) {
-                        int sx, sy, len;
-                        line_geom(dir, l, W, H, sx, sy, len);
-                        longest = std::max(longest, len);
+        std::vector<int> cls;
+        int64_t cls_cells = 0;
+        for (int i = 0; i < n; i++)
+            if (split_m(pairs[i].depth) == mc) {
+                cls.push_back(i);
+                cls_cells += (int64_t)pairs[i].width * pairs[i].height * pairs[i].depth;
+            }
+        if (cls.empty()) continue;
+        // K consecutive groups of the class's pairs with about equal cells:
+        // group k ends where the running total reaches k+1 K-ths, leaving at
+        // least one pair to each later group
+        const int K = std::min((int)cls.size(), batch_segments());
+        size_t c0 = 0;
+        int64_t acc = 0;
+        for (int k = 0; k < K; k++) {
+            size_t c1 = c0;
+            const int64_t goal = cls_cells * (k + 1) / K;
+            const size_t max_end = cls.size() - (size_t)(K - k - 1);
+            do {
+                const tdvh_pair &q = pairs[cls[c1]];
+                acc += (int64_t)q.width * q.height * q.depth;
+                c1++;
+            } while (c1 < max_end && acc < goal);
+            tdvh_batch::Seg sg{mc, 0, 0, 0, 0, 0};
+            std::vector<std::pair<int, BItem>> lines;
+            for (size_t c = c0; c < c1; c++) {
+                const int i = cls[c];
+                const int W = pairs[i].width, H = pairs[i].height;
+                for (int dir = 0; dir < 8; dir++) {
+                    const int nl = dir_lines(dir, W, H);
+                    for (int l0 = 0; l0 < nl; l0 += 4) {
+                        int longest = 0;
+                        for (int l = l0; l < std::min(l0 + 4, nl); l++) {
+                            int sx, sy, len;
+                            line_geom(dir, l, W, H, sx, sy, len);
+                            longest = std::max(longest, len);
+                        }
+                        lines.push_back({longest, BItem{i, 1, dir, l0}});
                     }
-                    lines.push_back({longest, BItem{i, 1, dir, l0}});
                 }
             }
+            // pair-major, longest line first within a pair: the next pair's
+            // long lines start while this pair's short ones finish, and only
+            // about one pair's term planes are live in the caches at a time
+            std::stable_sort(lines.begin(), lines.end(),
+                             [](const std::pair<int, BItem> &x, const std::pair<int, BItem> &y) {
+                                 if (x.second.pair != y.second.pair)
+                                     return x.second.pair < y.second.pair;
+                                 return x.first > y.first;
+                             });
+            sg.line_begin = (int)items.size();
+            for (auto &l : lines) items.push_back(l.second);
+            sg.line_count = (int)items.size() - sg.line_begin;
+            sg.sum_begin = (int)items.size();
+            if (mc == 1 || mc == 2 || mc == 4 || mc == 8) {
+                // one item per pair: {pair, 3, first wave, waves} (wide sum)
+                int wave = 0;
+                for (size_t c = c0; c < c1; c++) {
+                    const int i = cls[c];
+                    const long npix = (long)pairs[i].width * pairs[i].height;
+                    const int nw = (int)((npix * 2 * mc + 63) / 64);
+                    items.push_back({i, 3, wave, nw});
+                    wave += nw;
+                }
+                sg.sum_waves = wave;
+            } else {
+                for (size_t c = c0; c < c1; c++) {
+                    const int i = cls[c];
+                    const long npix = (long)pairs[i].width * pairs[i].height;
+                    for (long p = 0; p < npix; p += SUM_CHUNK)
+                        items.push_back({i, 2, (int)p, (int)std::min<long>(SUM_CHUNK, npix - p)});
+                }
+            }
+            sg.sum_count = (int)items.size() - sg.sum_begin;
+            plan.segs.push_back(sg);
+            c0 = c1;
         }
-        std::stable_sort(lines.begin(), lines.end(),
-                         [](const std::pair<int, BItem> &x, const std::pair<int, BItem> &y) {
-                             return x.first > y.first;
-                         });
-        plan.line_begin[mc] = (int)items.size();
-        for (auto &l : lines) items.push_back(l.second);
-        plan.line_count[mc] = (int)items.size() - plan.line_begin[mc];
-        plan.sum_begin[mc] = (int)items.size();
-        for (int i = 0; i < n; i++) {
-            if (split_m(pairs[i].depth) != mc) continue;
-            const long npix = (long)pairs[i].width * pairs[i].height;
-            for (long p = 0; p < npix; p += SUM_CHUNK)
-                items.push_back({i, 2, (int)p, (int)std::min<long>(SUM_CHUNK, npix - p)});
-        }
-        plan.sum_count[mc] = (int)items.size() - plan.sum_begin[mc];
     }
     const size_t pairs_off = total;
     total = align(total + sizeof(BPair) * n);
@@ -227,11 +335,12 @@ extern "C" int tdvh_batch_create(tdvh_ctx *ctx, const tdvh_pair *pairs, int n,
     b->d_items = (BItem *)(base + items_off);
     b->bt_begin = plan.bt_begin;
     b->bt_count = plan.bt_count;
-    for (int mc = 0; mc <= 8; mc++) {
-        b->line_begin[mc] = plan.line_begin[mc];
-        b->line_count[mc] = plan.line_count[mc];
-        b->sum_begin[mc] = plan.sum_begin[mc];
-        b->sum_count[mc] = plan.sum_count[mc];
+    b->segs = plan.segs;
+    for (int mc = 1; mc <= 8; mc++) b->class_buf[mc] = true;
+    for (int i = 0; i < n; i++) {
+        const int mc = bp[i].m;
+        if (bp[i].a.D < 32 * mc) b->class_pad[mc] = true;
+        if (!buf_lines_fit(bp[i].a)) b->class_buf[mc] = false;
     }
     b->cells = cells;
     if (hipMemcpy(b->d_pairs, bp.data(), sizeof(BPair) * n, hipMemcpyHostToDevice) != hipSuccess ||
@@ -256,8 +365,12 @@ extern "C" int tdvh_batch_destroy(tdvh_batch *b)
     if (b->graph_stream) (void)hipStreamDestroy(b->graph_stream);
     for (int mc = 0; mc <= 8; mc++) {
         if (b->branch[mc]) (void)hipStreamDestroy(b->branch[mc]);
+        if (b->sum_branch[mc]) (void)hipStreamDestroy(b->sum_branch[mc]);
         if (b->join[mc]) (void)hipEventDestroy(b->join[mc]);
+        if (b->sum_join[mc]) (void)hipEventDestroy(b->sum_join[mc]);
     }
+    for (hipEvent_t e : b->seg_done)
+        if (e) (void)hipEventDestroy(e);
     if (b->fork) (void)hipEventDestroy(b->fork);
     (void)hipFree(b->mem);
     delete b;
@@ -283,70 +396,121 @@ extern "C" int tdvh_batch_config(tdvh_batch *b, int lines_per_wave, int waves_pe
 }
 
 template <int M>
-static void launch_class(tdvh_batch *b, hipStream_t s)
+static void launch_lines_seg(tdvh_batch *b, const tdvh_batch::Seg &sg, hipStream_t s)
 {
     const int threads = 64 * b->waves_per_group, wpb = b->waves_per_group;
-    const int nl = b->line_count[M], ns = b->sum_count[M];
-    const BItem *li = b->d_items + b->line_begin[M], *si = b->d_items + b->sum_begin[M];
-    if (nl) {
-        prof_begin(s);
+    const int nl = sg.line_count;
+    if (!nl) return;
+    const BItem *li = b->d_items + sg.line_begin;
+    const int blocks = (nl + wpb - 1) / wpb;
+    prof_begin(s);
+    if (b->class_buf[M]) {
+        // prefetch 8 only where its ring fits two waves per SIMD
+        constexpr int PF8 = M <= 4 ? 8 : 4;
+        if (b->class_pad[M]) {
+            if (b->prefetch == 8)
+                batch_lines_buf_kernel<M, true, PF8><<<blocks, threads, 0, s>>>(b->d_pairs, li, nl,
+                                                                                b->prio);
+            else
+                batch_lines_buf_kernel<M, true, 4><<<blocks, threads, 0, s>>>(b->d_pairs, li, nl,
+                                                                              b->prio);
+        } else {
+            if (b->prefetch == 8)
+                batch_lines_buf_kernel<M, false, PF8><<<blocks, threads, 0, s>>>(b->d_pairs, li,
+                                                                                 nl, b->prio);
+            else
+                batch_lines_buf_kernel<M, false, 4><<<blocks, threads, 0, s>>>(b->d_pairs, li, nl,
+                                                                               b->prio);
+        }
+    } else {
         if (b->prefetch == 8)
-            batch_lines_kernel<M, 8><<<(nl + wpb - 1) / wpb, threads, 0, s>>>(b->d_pairs, li, nl,
-                                                                            b->prio);
+            batch_lines_kernel<M, 8><<<blocks, threads, 0, s>>>(b->d_pairs, li, nl, b->prio);
         else
-            batch_lines_kernel<M, 4><<<(nl + wpb - 1) / wpb, threads, 0, s>>>(b->d_pairs, li, nl,
-                                                                            b->prio);
-        (void)check_launch("batch_lines_kernel", s);
+            batch_lines_kernel<M, 4><<<blocks, threads, 0, s>>>(b->d_pairs, li, nl, b->prio);
     }
-    if (ns) {
-        prof_begin(s);
-        batch_sum_kernel<M><<<(ns + 3) / 4, 256, 0, s>>>(b->d_pairs, si, ns);
-        (void)check_launch("batch_sum_kernel", s);
-    }
+    (void)check_launch("batch_lines_kernel", s);
 }
 
-static int launch_class_m(tdvh_batch *b, int m, hipStream_t s)
+template <int M>
+static void launch_sum_seg(tdvh_batch *b, const tdvh_batch::Seg &sg, hipStream_t s)
 {
-    switch (m) {
-    case 1: launch_class<1>(b, s); break;
-    case 2: launch_class<2>(b, s); break;
-    case 3: launch_class<3>(b, s); break;
-    case 4: launch_class<4>(b, s); break;
-    case 5: launch_class<5>(b, s); break;
-    case 6: launch_class<6>(b, s); break;
-    case 7: launch_class<7>(b, s); break;
-    default: launch_class<8>(b, s); break;
-    }
-    return check_launch("batch_class", s);
+    const int ns = sg.sum_count;
+    if (!ns) return;
+    const BItem *si = b->d_items + sg.sum_begin;
+    prof_begin(s);
+    if constexpr (M == 1 || M == 2 || M == 4 || M == 8)
+        batch_sum_wide_kernel<M><<<(sg.sum_waves + 3) / 4, 256, 0, s>>>(b->d_pairs, si, ns,
+                                                                        sg.sum_waves);
+    else
+        batch_sum_kernel<M><<<(ns + 3) / 4, 256, 0, s>>>(b->d_pairs, si, ns);
+    (void)check_launch("batch_sum_kernel", s);
 }
 
-// Enqueues the batch on `s`. With fork != 0 (graph capture) the depth
-// classes run on branch streams forked from and joined back into `s`.
+static int launch_seg(tdvh_batch *b, const tdvh_batch::Seg &sg, bool sum, hipStream_t s)
+{
+    switch (sg.mc) {
+#define CASE(m)                                                                                    \
+    case m:                                                                                        \
+        if (sum) launch_sum_seg<m>(b, sg, s);                                                      \
+        else launch_lines_seg<m>(b, sg, s);                                                        \
+        break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+    default: break;
+    }
+    return check_launch(sum ? "batch_sum" : "batch_lines", s);
+}
+
+// Enqueues the batch on `s`. With fork (graph capture) every depth class
+// runs on two branch streams forked from and joined back into `s`: one scans
+// the lines of its segments in order, the other sums each segment as soon as
+// its lines are done, overlapping the next segment's scans (VALU-bound) with
+// this segment's plane reads (HBM-bound).
 static int enqueue_batch(tdvh_batch *b, hipStream_t s, bool fork)
 {
-    // the lines kernels of a class use lane blocks of 4 lines per wave with
-    // waves_per_group waves per workgroup (the sum kernels keep 4)
     prof_begin(s);
     batch_bt_kernel<<<(b->bt_count + 3) / 4, 256, 0, s>>>(b->d_pairs, b->d_items + b->bt_begin,
                                                           b->bt_count);
     TDVH_TRY(check_launch("batch_bt_kernel", s));
     if (!fork) {
-        for (int mc = 1; mc <= 8; mc++)
-            if (b->line_count[mc] || b->sum_count[mc]) TDVH_TRY(launch_class_m(b, mc, s));
+        for (const auto &sg : b->segs) {
+            TDVH_TRY(launch_seg(b, sg, false, s));
+            TDVH_TRY(launch_seg(b, sg, true, s));
+        }
         return TDVH_OK;
     }
     if (!b->fork) TDVH_CHECK_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    if (b->seg_done.size() < b->segs.size()) b->seg_done.resize(b->segs.size(), nullptr);
     TDVH_CHECK_HIP(hipEventRecord(b->fork, s));
     for (int mc = 1; mc <= 8; mc++) {
-        if (!b->line_count[mc] && !b->sum_count[mc]) continue;
+        bool any = false;
+        for (const auto &sg : b->segs) any |= sg.mc == mc;
+        if (!any) continue;
         if (!b->branch[mc])
             TDVH_CHECK_HIP(hipStreamCreateWithFlags(&b->branch[mc], hipStreamNonBlocking));
+        if (!b->sum_branch[mc])
+            TDVH_CHECK_HIP(hipStreamCreateWithFlags(&b->sum_branch[mc], hipStreamNonBlocking));
         if (!b->join[mc])
             TDVH_CHECK_HIP(hipEventCreateWithFlags(&b->join[mc], hipEventDisableTiming));
-        TDVH_CHECK_HIP(hipStreamWaitEvent(b->branch[mc], b->fork, 0));
-        TDVH_TRY(launch_class_m(b, mc, b->branch[mc]));
-        TDVH_CHECK_HIP(hipEventRecord(b->join[mc], b->branch[mc]));
+        if (!b->sum_join[mc])
+            TDVH_CHECK_HIP(hipEventCreateWithFlags(&b->sum_join[mc], hipEventDisableTiming));
+        hipStream_t ls = b->branch[mc], ss = b->sum_branch[mc];
+        TDVH_CHECK_HIP(hipStreamWaitEvent(ls, b->fork, 0));
+        TDVH_CHECK_HIP(hipStreamWaitEvent(ss, b->fork, 0));
+        for (size_t k = 0; k < b->segs.size(); k++) {
+            const auto &sg = b->segs[k];
+            if (sg.mc != mc) continue;
+            if (!b->seg_done[k])
+                TDVH_CHECK_HIP(hipEventCreateWithFlags(&b->seg_done[k], hipEventDisableTiming));
+            TDVH_TRY(launch_seg(b, sg, false, ls));
+            TDVH_CHECK_HIP(hipEventRecord(b->seg_done[k], ls));
+            TDVH_CHECK_HIP(hipStreamWaitEvent(ss, b->seg_done[k], 0));
+            TDVH_TRY(launch_seg(b, sg, true, ss));
+        }
+        TDVH_CHECK_HIP(hipEventRecord(b->join[mc], ls));
+        TDVH_CHECK_HIP(hipEventRecord(b->sum_join[mc], ss));
         TDVH_CHECK_HIP(hipStreamWaitEvent(s, b->join[mc], 0));
+        TDVH_CHECK_HIP(hipStreamWaitEvent(s, b->sum_join[mc], 0));
     }
     return TDVH_OK;
 }

@@ -644,16 +644,14 @@ __device__ __forceinline__ uint32_t group_min_u32(uint32_t v)
 // S = n*C + sum of E (invalid where C is), then the key (S << 9 | d)
 // reduced over the group (S <= 4336 and d < 512 keep keys ordered).
 template <int M, bool FUSED>
-__global__ __launch_bounds__(256) void sgm_sum_wide_kernel(const SgmArgs a, int nplanes,
-                                                           uint32_t *__restrict__ S,
-                                                           int16_t *__restrict__ disp, int pitch)
+__device__ __forceinline__ void sgm_sum_pixel_wide(const SgmArgs &a, int nplanes,
+                                                   uint32_t *__restrict__ S,
+                                                   int16_t *__restrict__ disp, int pitch, long p,
+                                                   int t)
 {
     static_assert(M == 1 || M == 2 || M == 4 || M == 8, "power-of-two word counts");
     constexpr int G = 2 * M;
     const int W = a.W, H = a.H;
-    const int t = threadIdx.x % G;
-    const long p = ((long)blockIdx.x * 256 + threadIdx.x) / G;
-    if (p >= (long)W * H) return; // whole groups leave together (256 % G == 0)
     const int y = (int)(p / W), x = (int)(p - (long)y * W);
     const int w0 = 8 * t;
     uint32_t c[8];
@@ -726,6 +724,17 @@ __global__ __launch_bounds__(256) void sgm_sum_wide_kernel(const SgmArgs a, int 
     }
     key = group_min_u32<G>(key);
     if (t == 0) disp[(size_t)y * pitch + x] = (int16_t)(key & 0x1FFu);
+}
+
+template <int M, bool FUSED>
+__global__ __launch_bounds__(256) void sgm_sum_wide_kernel(const SgmArgs a, int nplanes,
+                                                           uint32_t *__restrict__ S,
+                                                           int16_t *__restrict__ disp, int pitch)
+{
+    constexpr int G = 2 * M;
+    const long p = ((long)blockIdx.x * 256 + threadIdx.x) / G;
+    if (p >= (long)a.W * a.H) return; // whole groups leave together (256 % G == 0)
+    sgm_sum_pixel_wide<M, FUSED>(a, nplanes, S, disp, pitch, p, threadIdx.x % G);
 }
 
 template <int M, bool FUSED, bool EOUT>
