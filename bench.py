@@ -31,8 +31,9 @@ def b_alg(W, H, D):
 
 
 def measured_traffic(kernel):
-    """HBM bytes per dispatch of `kernel` from the committed PMC summary
-    (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel calibration), or None."""
+    """HBM bytes per dispatch of the kernel the library profiles as `kernel`,
+    from the committed PMC summary (FETCH_SIZE / WRITE_SIZE scaled by the
+    copy-kernel calibration), or None."""
     path = os.path.join(ROOT, "profiles", "traffic_sgm_fused_720p_d128.txt")
     try:
         rows, scale = {}, {}
@@ -44,12 +45,10 @@ def measured_traffic(kernel):
                 scale[parts[0]] = float(parts[1])
             else:
                 rows[parts[0]] = (float(parts[1]), float(parts[2]))
-        for name, (f, w) in rows.items():
-            if kernel in name:
-                return int((f * scale["scale_fetch"] + w * scale["scale_write"]) * 1024)
+        f, w = rows[kernel]
+        return int((f * scale["scale_fetch"] + w * scale["scale_write"]) * 1024)
     except (OSError, KeyError, ValueError, IndexError):
-        pass
-    return None
+        return None
 
 
 def cpu_model():
@@ -62,12 +61,22 @@ def cpu_model():
     return "unknown"
 
 
-def bench_batched(tv, synth, torch, ctx, reps):
-    """64 mixed pairs (320x240..1920x1080, D in {32,64,128,256} clamped to
-    W/4, seeds 1000+i): one persistent launch replayed from a hipGraph,
+MIX_WORKLOAD = ("64 pairs, sizes {320x240,640x480,1280x720,1920x1080}, D in {32,64,128,256} "
+                "clamped to W/4, seeds 1000+i; BT+SGM8+WTA")
+HD_WORKLOAD = "16 pairs 1280x720, D=128, 12 rectangles, seeds 3000+i; BT+SGM8+WTA"
+
+
+def hd_specs(n=16):
+    return [dict(width=1280, height=720, dmax=127, nrect=12, seed=3000 + i, depth=128)
+            for i in range(n)]
+
+
+def bench_batched(tv, synth, torch, ctx, reps, specs=None, workload=MIX_WORKLOAD):
+    """A batch of pairs (default: 64 mixed pairs, 320x240..1920x1080, D in
+    {32,64,128,256} clamped to W/4, seeds 1000+i) replayed from a hipGraph,
     against the plain per-pair call over the same pairs."""
     from thundervision_amd import autotune
-    specs = synth.batch_specs(64)
+    specs = specs if specs is not None else synth.batch_specs(64)
     dev = torch.device("cuda:0")
     pairs = []
     for s in specs:
@@ -96,8 +105,7 @@ def bench_batched(tv, synth, torch, ctx, reps):
     plain_round()
     torch.cuda.synchronize()
     exact = all(torch.equal(o, d) for o, (_, _, _, d) in zip(outs, pairs))
-    return {"workload": "64 pairs, sizes {320x240,640x480,1280x720,1920x1080}, D in "
-                        "{32,64,128,256} clamped to W/4, seeds 1000+i; BT+SGM8+WTA",
+    return {"workload": workload,
             "cells": cells, "ms_per_batch": round(ms_graph, 3),
             "value": round(cells / (ms_graph * 1e-3) / 1e6, 1), "unit": "MDE/s",
             "plain_per_pair_ms": round(ms_plain, 3),
@@ -108,12 +116,14 @@ def bench_batched(tv, synth, torch, ctx, reps):
             "bit_exact_vs_plain": exact}
 
 
-def bench_served(torch, duration=1.0, rate=600.0, frames_per_stream=4, slots=8):
+def bench_served(torch, duration=1.0, rate=600.0, frames_per_stream=4, slots=4):
     """16 streams (4 each at 640x480/D64, 1280x720/D128, 1280x720/D192,
     1920x1080/D256), Poisson arrivals at `rate` pairs/s in total, frames of
     stream s from the generator with seed 2000+s and drifting rectangles.
     Continuous batching over `slots` concurrent slots under a 64 GB budget,
-    against one slot (the plain per-pair call) at the same load."""
+    with at most 1.3 largest frames' cells in flight and 3 ms of backfill past
+    a blocked head frame, against one slot (the plain per-pair call) at the
+    same load."""
     from thundervision_amd import serve, synth
     dev = torch.device("cuda:0")
     shapes = serve.SERVED_STREAMS
@@ -135,18 +145,21 @@ def bench_served(torch, duration=1.0, rate=600.0, frames_per_stream=4, slots=8):
             disp = torch.empty((H, W), dtype=torch.int16, device=dev)
             frames.append(serve.Frame(t, s, k, W, H, D, (L, R, disp)))
         ex = serve.GpuExecutor(nslots)
-        sch = serve.Scheduler(ex, nslots, 64 << 30)
-        # warm the slots' scratch with the largest frame shape
-        for slot in range(nslots):
-            f = max(frames, key=lambda f: f.cells)
-            ex.launch(f, slot)
-        torch.cuda.synchronize()
+        big = max(W * H * D for W, H, D in shapes)
+        if nslots > 1:
+            sch = serve.Scheduler(ex, nslots, 64 << 30, cell_budget=1.3 * big, backfill_s=0.003)
+        else:
+            sch = serve.Scheduler(ex, nslots, 64 << 30)
+        # every slot runs every frame shape once before the clock starts
+        ex.warm(frames)
         ex.start()
         done = sch.run(frames)
         torch.cuda.synchronize()
         st = serve.latency_stats(done)
         out[name] = {k: round(v, 3) for k, v in st.items()}
         out[name]["slots"] = nslots
+        out[name]["cell_budget"] = sch.cell_budget
+        out[name]["backfill_ms"] = None if sch.backfill_s is None else sch.backfill_s * 1e3
         out[name]["max_concurrent"] = sch.max_concurrent
         out[name]["peak_inflight_GB"] = round(sch.peak_in_use / 2**30, 2)
     out["offered_MDE/s"] = round(sum(W * H * D for t, s, k in sched
@@ -236,9 +249,11 @@ def main():
     achieved = b_alg(W, H, D) / (dom_ms * 1e-3) / 1e9
     pipeline_ms = sum(v[0] / v[1] for v in table.values())
 
-    batched = None
+    batched = batched_hd = None
     if not args.no_batched:
         batched = bench_batched(tv, synth, torch, ctx, max(3, args.steps // 5))
+        batched_hd = bench_batched(tv, synth, torch, ctx, max(3, args.steps // 5),
+                                   specs=hd_specs(), workload=HD_WORKLOAD)
 
     served = None
     if not args.no_served:
@@ -289,6 +304,7 @@ def main():
                        "ms_per_step": round(sec_other * 1e3, 4)},
         "kernels_ms": {k: round(v[0] / v[1], 4) for k, v in table.items()},
         "batched": batched,
+        "batched_720p": batched_hd,
         "served": served,
         "cpu_baseline": cpu,
     }

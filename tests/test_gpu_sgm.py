@@ -181,7 +181,11 @@ def test_errors_are_reported(tv, dev):
 
 @pytest.mark.parametrize("fusion", [0, 1])
 @pytest.mark.parametrize("W,H,D,p2", [(97, 40, 80, 32), (65, 33, 33, 300), (300, 20, 256, 32),
-                                      (40, 70, 160, 7), (31, 17, 1, 32)])
+                                      (40, 70, 160, 7), (31, 17, 1, 32),
+                                      # long sides scanned 64 lanes per line (m = 4, 8;
+                                      # horizontal and vertical; with padding cells)
+                                      (200, 64, 100, 32), (50, 200, 128, 32),
+                                      (33, 300, 250, 20), (260, 90, 256, 32)])
 def test_stereo_shapes_and_paths(tv, dev, fusion, W, H, D, p2):
     L, R = pair(W, H, max(D - 1, 0), 3 * W + 7 * H)
     want = oracle.wta(oracle.sgm_std(oracle.cost_bt2(L, R, D), 8, p2))

@@ -58,6 +58,59 @@ def test_budget_and_slots_are_respected():
         assert idx == sorted(idx)
 
 
+def test_cell_budget_keeps_large_frames_apart():
+    sched = serve.poisson_schedule(16, 600.0, 0.5)
+    frames = frames_from(sched, serve.SERVED_STREAMS)
+    big = 1920 * 1080 * 256
+    ex = FakeExecutor(rate=2e11)
+    sch = serve.Scheduler(ex, slots=4, budget_bytes=64 << 30, cell_budget=1.3 * big)
+    seen = []
+    launch = ex.launch
+
+    def watch(f, slot):
+        seen.append(sch.in_flight_cells)
+        launch(f, slot)
+
+    ex.launch = watch
+    done = sch.run(frames)
+    assert len(done) == len(frames)
+    # with more than one frame in flight the cells stay within the budget;
+    # a frame larger than the budget would still start on an idle device
+    assert max(seen) <= 1.3 * big
+    assert sch.max_concurrent >= 2
+
+
+def test_backfill_passes_a_blocked_head_only_for_a_while():
+    # a large frame in flight; a second large frame (blocked by the cell
+    # budget) heads the queue; small frames behind it may start while the
+    # head has waited less than backfill_s, not after
+    big = (1920, 1080, 256)
+    small = (320, 240, 32)
+    frames = [serve.Frame(0.0, 0, 0, *big), serve.Frame(0.00005, 1, 0, *big),
+              serve.Frame(0.0001, 2, 0, *small), serve.Frame(0.02, 3, 0, *small)]
+    ex = FakeExecutor(rate=2e10)  # a large frame takes ~26 ms
+    sch = serve.Scheduler(ex, slots=4, budget_bytes=64 << 30,
+                          cell_budget=1.3 * 1920 * 1080 * 256, backfill_s=0.005)
+    done = {(f.stream, f.index): f for f in sch.run(frames)}
+    # the early small frame ran next to the first large one
+    assert done[(2, 0)].t_start < done[(0, 0)].t_done
+    # the late small frame arrived after the backfill window: it waits for
+    # the blocked large frame to start first
+    assert done[(3, 0)].t_start >= done[(1, 0)].t_start
+
+
+def test_gc_is_paused_while_serving_and_restored():
+    import gc
+    ex = FakeExecutor()
+    states = []
+    launch = ex.launch
+    ex.launch = lambda f, slot: (states.append(gc.isenabled()), launch(f, slot))
+    sch = serve.Scheduler(ex, slots=2, budget_bytes=64 << 30)
+    assert gc.isenabled()
+    sch.run([serve.Frame(0.0, 0, 0, 64, 48, 16), serve.Frame(0.001, 1, 0, 64, 48, 16)])
+    assert states == [False, False] and gc.isenabled()
+
+
 def test_oversized_frame_is_rejected():
     ex = FakeExecutor()
     sch = serve.Scheduler(ex, slots=2, budget_bytes=1000)

@@ -17,6 +17,7 @@
 // and each writes its own path-cost plane; a second kernel sums the planes
 // and takes the lowest-index argmin (wtacpu.cpp:20-33).
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <string>
 #include <vector>
@@ -134,9 +135,13 @@ static int line_order(tdvh_ctx *ctx, int W, int H, int dir_mask, const uint32_t 
                      [](const std::pair<int, uint32_t> &x, const std::pair<int, uint32_t> &y) {
                          return x.first > y.first;
                      });
+    // one table per shape and mask (a context serving several shapes keeps
+    // them all, so steady-state calls never upload or synchronise)
+    char name[64];
+    snprintf(name, sizeof name, "sgm_line_order:%dx%d:%02x", W, H, dir_mask);
     void *buf = nullptr;
-    TDVH_TRY(ctx->scratch("sgm_line_order", g.size() * 4, &buf));
-    tdvh_ctx::Slot &sl = ctx->slot("sgm_line_order");
+    TDVH_TRY(ctx->scratch(name, g.size() * 4, &buf));
+    tdvh_ctx::Slot &sl = ctx->slot(name);
     const uint64_t tag = ((uint64_t)W << 32) | ((uint64_t)H << 8) | (uint64_t)dir_mask;
     if (sl.tag != tag) {
         std::vector<uint32_t> h(g.size());

@@ -12,7 +12,7 @@ The scheduling logic is independent of the executor, so it is tested on the
 CPU with a fake executor; GpuExecutor runs tdvh_stereo_bt_sgm.
 """
 import collections
-import heapq
+import gc
 import time
 
 BT_TERM_WORDS = 3
@@ -55,17 +55,46 @@ class Scheduler:
     executor.launch(frame, slot) starts a frame; executor.poll(slot) returns
     the completion time (seconds on the schedule clock) or None;
     executor.now() is the schedule clock.
+
+    Admission is in arrival order, needs a free slot and memory within
+    budget_bytes, and, with cell_budget, keeps the cells (W*H*D) of the
+    frames in flight within that budget, so that large frames do not share
+    the device with each other (a frame always starts on an idle device).
+    With backfill_s, frames behind a head frame that does not fit yet may
+    start first while the head has waited less than backfill_s seconds.
     """
 
-    def __init__(self, executor, slots, budget_bytes):
+    def __init__(self, executor, slots, budget_bytes, cell_budget=None, backfill_s=None):
         self.ex = executor
         self.nslots = slots
         self.budget = budget_bytes
+        self.cell_budget = cell_budget
+        self.backfill_s = backfill_s
         self.in_use = 0
+        self.in_flight_cells = 0
         self.peak_in_use = 0
         self.max_concurrent = 0
 
+    def _fits(self, f, busy):
+        if self.in_use + f.footprint > self.budget:
+            return False
+        if self.cell_budget is not None and busy:
+            return self.in_flight_cells + f.cells <= self.cell_budget
+        return True
+
     def run(self, frames, spin_sleep=0.0):
+        """Serves `frames`; Python's cyclic garbage collector is paused while
+        the loop runs (a full collection takes tens of milliseconds, which
+        would land on some frame's latency)."""
+        was_enabled = gc.isenabled()
+        gc.disable()
+        try:
+            return self._run(frames, spin_sleep)
+        finally:
+            if was_enabled:
+                gc.enable()
+
+    def _run(self, frames, spin_sleep):
         pending = sorted(frames, key=lambda f: (f.t_arrival, f.stream, f.index))
         queue = collections.deque()
         free = list(range(self.nslots))
@@ -81,13 +110,25 @@ class Scheduler:
                                      % (f.width, f.height, f.depth))
                 queue.append(f)
                 i += 1
-            # admit in arrival order while a slot is free and memory allows
-            while queue and free and self.in_use + queue[0].footprint <= self.budget:
-                f = queue.popleft()
+            # admit in arrival order while a slot is free and the budgets
+            # allow (optionally backfilling past a blocked head frame)
+            while queue and free:
+                pick = None
+                for j, f in enumerate(queue):
+                    if self._fits(f, busy):
+                        pick = j
+                        break
+                    if self.backfill_s is None or now - queue[0].t_arrival >= self.backfill_s:
+                        break
+                if pick is None:
+                    break
+                f = queue[pick]
+                del queue[pick]
                 slot = free.pop()
                 f.slot = slot
                 f.t_start = now
                 self.in_use += f.footprint
+                self.in_flight_cells += f.cells
                 self.peak_in_use = max(self.peak_in_use, self.in_use)
                 self.ex.launch(f, slot)
                 busy[slot] = f
@@ -99,6 +140,7 @@ class Scheduler:
                     f = busy.pop(slot)
                     f.t_done = t
                     self.in_use -= f.footprint
+                    self.in_flight_cells -= f.cells
                     free.append(slot)
                     done.append(f)
                     progressed = True
@@ -139,6 +181,25 @@ class GpuExecutor:
         self.events = [None] * slots
         self.t0_host = None
         self.t0_event = None
+
+    def warm(self, frames):
+        """Runs one frame of every shape in `frames` on every slot and polls
+        them to completion, so that scratch, per-shape tables and the event
+        and timing paths are initialised before start()."""
+        torch = self.torch
+        firsts = {}
+        for f in frames:
+            firsts.setdefault((f.width, f.height, f.depth), f)
+        shapes = sorted(firsts.values(), key=lambda f: -f.cells)
+        self.start()
+        for f in shapes:
+            for slot in range(len(self.streams)):
+                self.launch(f, slot)
+            for slot in range(len(self.streams)):
+                while self.poll(slot) is None:
+                    time.sleep(0)
+        torch.cuda.synchronize()
+        gc.collect()
 
     def start(self):
         torch = self.torch
