@@ -227,6 +227,64 @@ typedef struct {
     int16_t *disp;               /* device, row pitch = width */
 } tdvh_pair;
 
+/* ---- around the path: filters, formats, transfers, rectification, 3D ------- */
+
+/* Gaussian 5x5 prefilter of a uint8 image: binomial kernel [1 4 6 4 1]^2/256,
+ * edge-replicated border, rounded half up. Replaces MedianFilterCPU::updateImpl
+ * (medianfiltercpu.cpp:7-19, cvSmooth CV_GAUSSIAN 5), the "median filter" stage
+ * DevStereoMatcher puts in front of the matching cost
+ * (devstereomatcher.cpp:22-29). dst must not alias src. */
+int tdvh_prefilter_gauss5(tdvh_ctx *ctx, const uint8_t *src, int src_pitch, int width,
+                          int height, uint8_t *dst, int dst_pitch, void *stream);
+
+/* Opening of int16 disparities: `iterations` 3x3 erosions, then as many 3x3
+ * dilations (a (2k+1)^2 square each way; outside pixels ignored). Replaces
+ * Dilate::updateImpl (dilate.cpp:12-22, cvErode/cvDilate with 3 iterations),
+ * which follows the matcher in Reconstruction (reconstruction.cpp:30-31).
+ * May run in place. */
+int tdvh_postfilter_open(tdvh_ctx *ctx, const int16_t *src, int src_pitch, int width,
+                         int height, int iterations, int16_t *dst, int dst_pitch, void *stream);
+
+enum { TDVH_RGB = 0, TDVH_BGR = 1 };
+
+/* 3- or 4-channel uint8 colour to gray, (4899 R + 9617 G + 1868 B + 8192) >> 14.
+ * Replaces misc::create8UGray (misc.cpp:10-16, cvCvtColor CV_RGB2GRAY). */
+int tdvh_rgb_to_gray(tdvh_ctx *ctx, const uint8_t *src, int src_pitch, int width, int height,
+                     int channels, int order, uint8_t *dst, int dst_pitch, void *stream);
+
+/* Rectification remap of a uint8 image: output pixel (x, y) samples src at
+ * (map_x, map_y)[y][x], bilinear with coordinates rounded to 1/32 pixel and
+ * integer weights summing to 1024 ((sum + 512) >> 10); neighbours outside the
+ * source read `border`. Replaces cvRemap in RectificationCV::update
+ * (rectificationcv.cpp:292-293). Maps are float, pitch in elements. */
+int tdvh_remap_u8(tdvh_ctx *ctx, const uint8_t *src, int src_pitch, int width, int height,
+                  const float *map_x, const float *map_y, int map_pitch, int out_width,
+                  int out_height, int border, uint8_t *dst, int dst_pitch, void *stream);
+
+/* Disparity to 3D: for pixel (x, y), [X Y Z w] = Q [x, H - y, disp * disp_scale, 1]
+ * (Q row-major 4x4, evaluated in double), xyz = (X, Y, Z) / w (0 where
+ * |w| <= FLT_EPSILON), Z *= z_scale. Output float triples, pitch in floats.
+ * Replaces CVReprojector::reproject (cvreprojector.cpp:11-26; the reference
+ * passes the d/D float disparity, i.e. disp_scale = 1/D, and z_scale = 24). */
+int tdvh_reproject(tdvh_ctx *ctx, const int16_t *disp, int disp_pitch, int width, int height,
+                   const float q[16], float disp_scale, float z_scale, float *xyz,
+                   int xyz_pitch, void *stream);
+
+/* Page-locked host memory for asynchronous transfers (hipHostMalloc). */
+int tdvh_host_alloc(size_t bytes, void **out);
+int tdvh_host_free(void *p);
+
+/* One asynchronous 2D copy (host<->device or device<->device, pitches in
+ * bytes). Replaces the per-row cudaMemcpy loops of FloatImageImpl::devMem and
+ * ::cpuMem (floatimage.cpp:84-91, 118-124). */
+int tdvh_copy_2d(tdvh_ctx *ctx, void *dst, size_t dst_pitch, const void *src,
+                 size_t src_pitch, size_t width_bytes, int rows, void *stream);
+
+/* Copies a densely packed cost volume to host memory in one transfer, in its
+ * own layout. Replaces LocalDSIMem::toCpuMem (dsimem.cpp:55-92), which issues
+ * one cudaMemcpy per (x, y). */
+int tdvh_dsi_download(tdvh_ctx *ctx, const tdvh_dsi *dsi, void *host, void *stream);
+
 typedef struct tdvh_batch tdvh_batch;
 
 /* Plans a batch (work queue, scratch) for `n` pairs. The pair descriptors

@@ -32,6 +32,9 @@ public:
     float *devMem();              // float copy on the device
     float *cpuMem();              // float copy on the host
     const uint8_t *devMemU8();    // uint8 gray copy on the device
+    // The uint8 plane on the device, to be written by a producer; the other
+    // views are refreshed from it on demand.
+    uint8_t *writableDevU8();
     int16_t *disparityDev();      // int16 disparity plane (allocated on first use)
     void disparityDepth(int depth); // the D of d/D for the float view
     int disparityDepth() const;

@@ -33,12 +33,49 @@ private:
     ReadWritePipe<FloatImage> m_wpipe;
 };
 
-// Matching cost -> optimizer -> host copy on one device process. The
-// reference's Gaussian prefilter (devstereomatcher.cpp:22-29) is not part of
-// this path; inputs go straight to the cost stage.
+// Gaussian 5x5 prefilter of an input image on the device (the reference's
+// "median filter" stage: MedianFilterCPU, medianfiltercpu.cpp:7-19, which
+// DevStereoMatcher uses; MedianFilterDev, medianfilterdev.cpp:11-27, is the
+// device stage of the same name). Runs tdvh_prefilter_gauss5 on the uint8
+// plane.
+class MedianFilterDev : public WorkUnit {
+public:
+    MedianFilterDev() { workName("Median filter device"); }
+    void input(ReadPipe<FloatImage> *p) { m_rpipe = p; }
+    ReadPipe<FloatImage> *output() { return &m_wpipe; }
+    bool update();
+
+private:
+    ReadPipe<FloatImage> *m_rpipe = nullptr;
+    ReadWritePipe<FloatImage> m_wpipe;
+};
+typedef MedianFilterDev MedianFilterCPU;
+
+// Erode x3 then dilate x3 (3x3 element) of a disparity map (dilate.cpp:12-22),
+// on the device via tdvh_postfilter_open on the int16 plane; the float d/D
+// view is refreshed from it.
+class Dilate : public WorkUnit {
+public:
+    Dilate(int iterations = 3) : m_iterations(iterations) { workName("Dilate"); }
+    void input(ReadPipe<FloatImage> *p) { m_rpipe = p; }
+    ReadPipe<FloatImage> *output() { return &m_wpipe; }
+    bool update();
+
+private:
+    int m_iterations;
+    ReadPipe<FloatImage> *m_rpipe = nullptr;
+    ReadWritePipe<FloatImage> m_wpipe;
+};
+
+// [prefilter ->] matching cost -> optimizer -> host copy on one device
+// process. As in the reference (devstereomatcher.cpp:6-49) both inputs go
+// through the Gaussian prefilter unless useMedianFilter(false) is set before
+// inputs().
 class DevStereoMatcher : public StereoMatcher {
 public:
     DevStereoMatcher(int device = 0);
+    void useMedianFilter(bool on) { m_useMedianfilter = on; }
+    bool useMedianFilter() const { return m_useMedianfilter; }
     void inputs(ReadPipe<FloatImage> *leftInput, ReadPipe<FloatImage> *rightInput);
     Process **processes() { return m_procs; }
     void setMatchingCost(std::shared_ptr<MatchingCost> mc) { m_matchCost = mc; }
@@ -57,6 +94,8 @@ public:
 private:
     std::shared_ptr<MatchingCost> m_matchCost;
     std::shared_ptr<Optimizer> m_optimizer;
+    MedianFilterDev m_medianFilter[2];
+    bool m_useMedianfilter = true;
     CpyImageToCPU m_cpy;
     DeviceProcess m_process;
     Process *m_procs[2];

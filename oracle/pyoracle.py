@@ -151,3 +151,50 @@ def fast_wta_ssd(L, R, D, radius=3):
 def stereo_bt_sgm(L, R, D, P1=8, P2=32):
     """BT2 -> 8-direction standard SGM -> lowest-index WTA."""
     return wta(sgm_std(cost_bt2(L, R, D), P1, P2))
+
+
+# ---- around the path (tdvref_io.c) ------------------------------------------
+
+def gauss5(img):
+    img = _img(img)
+    H, W = img.shape
+    out = np.empty_like(img)
+    lib().tdvref_gauss5_u8(_p(img), W, H, _p(out))
+    return out
+
+
+def open_disp(disp, iterations=3):
+    d = np.ascontiguousarray(disp, dtype=np.int16)
+    H, W = d.shape
+    out = np.empty_like(d)
+    lib().tdvref_open_i16(_p(d), W, H, iterations, _p(out))
+    return out
+
+
+def rgb_to_gray(img, order=0):
+    a = np.ascontiguousarray(img, dtype=np.uint8)
+    H, W, ch = a.shape
+    out = np.empty((H, W), np.uint8)
+    lib().tdvref_rgb_to_gray_u8(_p(a), W, H, ch, order, _p(out))
+    return out
+
+
+def remap(img, mapx, mapy, border=0):
+    img = _img(img)
+    H, W = img.shape
+    mx = np.ascontiguousarray(mapx, dtype=np.float32)
+    my = np.ascontiguousarray(mapy, dtype=np.float32)
+    Ho, Wo = mx.shape
+    out = np.empty((Ho, Wo), np.uint8)
+    lib().tdvref_remap_u8(_p(img), W, H, _p(mx), _p(my), Wo, Ho, border, _p(out))
+    return out
+
+
+def reproject(disp, Q, disp_scale=1.0, z_scale=24.0):
+    d = np.ascontiguousarray(disp, dtype=np.int16)
+    H, W = d.shape
+    q = np.ascontiguousarray(Q, dtype=np.float32).reshape(16)
+    out = np.empty((H, W, 3), np.float32)
+    lib().tdvref_reproject(_p(d), W, H, _p(q), ct.c_float(disp_scale), ct.c_float(z_scale),
+                           _p(out))
+    return out

@@ -136,6 +136,43 @@ void tdvref_dp(const uint32_t *C, int W, int H, int D, uint32_t p_occ,
 void tdvref_fast_wta_ssd(const uint8_t *L, const uint8_t *R, int W, int H,
                          int D, int radius, int16_t *disp);
 
+/* ---- around the path: filters, formats, rectification, reprojection ---------
+ * (tdvref_io.c). These restate the OpenCV calls the reference makes at the
+ * matcher boundary in the integer profile; OpenCV itself is not available, so
+ * they are pinned by closed forms, known answers and independent restatements
+ * (scipy) in the tests, not against OpenCV's outputs. */
+
+/* 5x5 binomial Gaussian ([1 4 6 4 1] x [1 4 6 4 1] / 256), edge-replicated
+ * border, rounded half up: cvSmooth(CV_GAUSSIAN, 5) of
+ * medianfiltercpu.cpp:7-19 on uint8 images. */
+void tdvref_gauss5_u8(const uint8_t *src, int W, int H, uint8_t *dst);
+
+/* `iterations` 3x3 erosions then `iterations` 3x3 dilations (neighbours
+ * outside the image ignored): cvErode / cvDilate with 3 iterations of
+ * dilate.cpp:12-22, on int16 disparities. */
+void tdvref_open_i16(const int16_t *src, int W, int H, int iterations, int16_t *dst);
+
+/* RGB (order 0) or BGR (order 1) uint8 pixels with `channels` 3 or 4 to
+ * gray: (4899 R + 9617 G + 1868 B + 8192) >> 14, the 14-bit fixed-point
+ * luma of cvCvtColor(CV_RGB2GRAY) on 8-bit data (misc.cpp:10-16). */
+void tdvref_rgb_to_gray_u8(const uint8_t *src, int W, int H, int channels, int order,
+                           uint8_t *dst);
+
+/* Bilinear remap (cvRemap, rectificationcv.cpp:292-293): output pixel i
+ * samples src at (mapx[i], mapy[i]); coordinates are rounded to 1/32 pixel,
+ * the four weights (32-ax)(32-ay), ax(32-ay), (32-ax)ay, ax*ay sum to 1024,
+ * result (sum + 512) >> 10; neighbours outside the image read `border`. */
+void tdvref_remap_u8(const uint8_t *src, int W, int H, const float *mapx, const float *mapy,
+                     int Wo, int Ho, int border, uint8_t *dst);
+
+/* Disparity to 3D (cvreprojector.cpp:11-26): pixel (x, y) with disparity
+ * value v = disp * disp_scale (float) maps to [X Y Z w] = Q [x, H - y, v, 1]
+ * evaluated in double in the order ((q0 x + q1 y') + q2 v) + q3 per row,
+ * then xyz = (X, Y, Z) * (1 / w) (all 0 when |w| <= FLT_EPSILON) stored as
+ * float, and Z *= z_scale (the reference uses 24); out is float [y][x][3]. */
+void tdvref_reproject(const int16_t *disp, int W, int H, const float Q[16], float disp_scale,
+                      float z_scale, float *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -197,14 +197,48 @@ static void test_gpu(const std::string &dir)
         m.setMatchingCost(std::make_shared<BirchfieldCostDev>(c.D));
         m.setOptimizer(std::make_shared<SemiGlobalDev>());
         FloatImage out;
+        CHECK(m.useMedianFilter()); // on by default, as devstereomatcher.cpp:6-12
         auto d = run_matcher(m, c, &out);
-        CHECK(same(d, slurp(dir + "/bt_sgm.i16"), "BT+SGM"));
+        CHECK(same(d, slurp(dir + "/bt_sgm_pf.i16"), "prefilter+BT+SGM"));
         const float *fv = out.cpuMem();
         bool ok = true;
         for (size_t i = 0; i < d.size(); i++) ok = ok && fv[i] == (float)d[i] / (float)c.D;
         CHECK(ok);
         CHECK(m.name() == "Birchfield Cost+Semi-global device");
         CHECK(m.matchcostBenchmark().probes() == 1 && m.optimizationBenchmark().probes() == 1);
+    }
+    { // without the prefilter, and the Dilate postfilter stage
+        DevStereoMatcher m;
+        m.useMedianFilter(false);
+        m.setMatchingCost(std::make_shared<BirchfieldCostDev>(c.D));
+        m.setOptimizer(std::make_shared<SemiGlobalDev>());
+        CHECK(same(run_matcher(m, c, nullptr), slurp(dir + "/bt_sgm.i16"), "BT+SGM raw"));
+        DevStereoMatcher m2;
+        m2.setMatchingCost(std::make_shared<BirchfieldCostDev>(c.D));
+        m2.setOptimizer(std::make_shared<SemiGlobalDev>());
+        Dilate dil;
+        ReadWritePipe<FloatImage> lp, rp;
+        m2.inputs(&lp, &rp);
+        dil.input(m2.output());
+        ProcessRunner runner(m2);
+        runner.run();
+        lp.write(FloatImage::FromGray8((const uint8_t *)c.L.data(), c.W, c.H));
+        rp.write(FloatImage::FromGray8((const uint8_t *)c.R.data(), c.W, c.H));
+        lp.finish();
+        rp.finish();
+        CHECK(dil.update());
+        FloatImage out;
+        CHECK(dil.output()->read(&out));
+        std::vector<int16_t> d(c.W * c.H);
+        checkHip(hipMemcpy(d.data(), out.disparityDev(), d.size() * 2, hipMemcpyDeviceToHost),
+                 "copy disparity");
+        runner.join();
+        CHECK(!runner.errorOcurred());
+        CHECK(same(d, slurp(dir + "/bt_sgm_pf_open.i16"), "prefilter+BT+SGM+Dilate"));
+        const float *fv = out.cpuMem();
+        bool ok = true;
+        for (size_t i = 0; i < d.size(); i++) ok = ok && fv[i] == (float)d[i] / (float)c.D;
+        CHECK(ok);
     }
     { // fused BT inside SGM
         SemiglobalLabMatcher m(c.D);
@@ -214,13 +248,13 @@ static void test_gpu(const std::string &dir)
         DevStereoMatcher m;
         m.setMatchingCost(std::make_shared<BirchfieldCostDev>(c.D));
         m.setOptimizer(std::make_shared<WTADev>());
-        CHECK(same(run_matcher(m, c, nullptr), slurp(dir + "/bt_wta.i16"), "BT+WTA"));
+        CHECK(same(run_matcher(m, c, nullptr), slurp(dir + "/bt_wta_pf.i16"), "BT+WTA"));
     }
     { // SSD + DP and SSD + WTA, FastWTAMatcher
         DevStereoMatcher m;
         m.setMatchingCost(std::make_shared<SSDDev>(c.D));
         m.setOptimizer(std::make_shared<DynamicProgDev>(20));
-        CHECK(same(run_matcher(m, c, nullptr), slurp(dir + "/ssd_dp.i16"), "SSD+DP"));
+        CHECK(same(run_matcher(m, c, nullptr), slurp(dir + "/ssd_dp_pf.i16"), "SSD+DP"));
         FastWTAMatcher fw(c.D);
         CHECK(same(run_matcher(fw, c, nullptr), slurp(dir + "/ssd_wta.i16"), "FastWTA"));
     }
@@ -230,7 +264,8 @@ static void test_gpu(const std::string &dir)
         fac.matchingCost(CommonStereoMatcherFactory::SSD);
         fac.optimization(CommonStereoMatcherFactory::WTA);
         std::unique_ptr<StereoMatcher> m(fac.createStereoMatcher());
-        CHECK(same(run_matcher(*m, c, nullptr), slurp(dir + "/ssd_wta.i16"), "factory SSD+WTA"));
+        CHECK(same(run_matcher(*m, c, nullptr), slurp(dir + "/ssd_wta_pf.i16"),
+                   "factory SSD+WTA"));
         fac.computeDev(CommonStereoMatcherFactory::CPU);
         bool threw = false;
         try {

@@ -34,9 +34,17 @@ def test_stage_chains_on_gpu(tmp_path):
     R.tofile(tmp_path / "right.u8")
     C = oracle.cost_bt2(L, R, D)
     oracle.wta(oracle.sgm_std(C, 8, 32)).tofile(tmp_path / "bt_sgm.i16")
-    oracle.wta(C).tofile(tmp_path / "bt_wta.i16")
     np.ascontiguousarray(C.transpose(1, 0, 2)).tofile(tmp_path / "bt_xyd.u16")
-    S = oracle.cost_ssd(L, R, D, 3)
-    oracle.dp(S, 20).tofile(tmp_path / "ssd_dp.i16")
-    oracle.wta(S).tofile(tmp_path / "ssd_wta.i16")
+    oracle.wta(oracle.cost_ssd(L, R, D, 3)).tofile(tmp_path / "ssd_wta.i16")
+    # DevStereoMatcher chains see the Gaussian-prefiltered pair
+    # (devstereomatcher.cpp:22-29); the fused matchers see the raw pair
+    Lf, Rf = oracle.gauss5(L), oracle.gauss5(R)
+    Cf = oracle.cost_bt2(Lf, Rf, D)
+    sgm_f = oracle.wta(oracle.sgm_std(Cf, 8, 32))
+    sgm_f.tofile(tmp_path / "bt_sgm_pf.i16")
+    oracle.open_disp(sgm_f, 3).tofile(tmp_path / "bt_sgm_pf_open.i16")
+    oracle.wta(Cf).tofile(tmp_path / "bt_wta_pf.i16")
+    Sf = oracle.cost_ssd(Lf, Rf, D, 3)
+    oracle.dp(Sf, 20).tofile(tmp_path / "ssd_dp_pf.i16")
+    oracle.wta(Sf).tofile(tmp_path / "ssd_wta_pf.i16")
     assert "gpu: ok" in run("gpu", str(tmp_path))

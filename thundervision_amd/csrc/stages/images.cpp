@@ -156,6 +156,15 @@ const uint8_t *FloatImage::devMemU8()
     return d;
 }
 
+uint8_t *FloatImage::writableDevU8()
+{
+    Impl &m = *m_impl;
+    uint8_t *d = (uint8_t *)m.dev8.get(m.dim.width() * m.dim.height());
+    m.dev8Valid = true;
+    m.devValid = m.hostValid = m.host8Valid = false;
+    return d;
+}
+
 int16_t *FloatImage::disparityDev()
 {
     const size_t n = m_impl->dim.width() * m_impl->dim.height();

@@ -22,10 +22,51 @@ DevStereoMatcher::DevStereoMatcher(int device) : m_process(device)
     m_procs[1] = nullptr;
 }
 
+bool MedianFilterDev::update()
+{
+    WriteGuard<ReadWritePipe<FloatImage>> guard(m_wpipe);
+    FloatImage in;
+    if (m_rpipe->read(&in)) {
+        const int W = (int)in.dim().width(), H = (int)in.dim().height();
+        FloatImage out = FloatImage::CreateDev(in.dim());
+        const uint8_t *src = in.devMemU8();
+        checkTdvh(tdvh_prefilter_gauss5(deviceContext(), src, W, W, H, out.writableDevU8(), W,
+                                        nullptr),
+                  "tdvh_prefilter_gauss5");
+        checkHip(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+        guard.write(out);
+    }
+    return guard.wasWrite();
+}
+
+bool Dilate::update()
+{
+    WriteGuard<ReadWritePipe<FloatImage>> guard(m_wpipe);
+    FloatImage img;
+    if (m_rpipe->read(&img)) {
+        const int W = (int)img.dim().width(), H = (int)img.dim().height();
+        int16_t *d = img.disparityDev();
+        checkTdvh(tdvh_postfilter_open(deviceContext(), d, W, W, H, m_iterations, d, W, nullptr),
+                  "tdvh_postfilter_open");
+        img.syncFromDisparity();
+        checkHip(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+        guard.write(img);
+    }
+    return guard.wasWrite();
+}
+
 void DevStereoMatcher::inputs(ReadPipe<FloatImage> *leftInput, ReadPipe<FloatImage> *rightInput)
 {
     if (!m_matchCost || !m_optimizer) throw Exception("DevStereoMatcher: stages not set");
-    m_matchCost->inputs(leftInput, rightInput);
+    if (m_useMedianfilter) {
+        m_medianFilter[0].input(leftInput);
+        m_medianFilter[1].input(rightInput);
+        m_matchCost->inputs(m_medianFilter[0].output(), m_medianFilter[1].output());
+        m_process.addWork(&m_medianFilter[0]);
+        m_process.addWork(&m_medianFilter[1]);
+    } else {
+        m_matchCost->inputs(leftInput, rightInput);
+    }
     m_optimizer->input(m_matchCost->output());
     m_cpy.input(m_optimizer->output());
     m_process.addWork(m_matchCost.get());

@@ -81,6 +81,15 @@ SIGNATURES = {
     "tdvh_disp_to_unit_float": (_I, [_VP, _VP, _I, _I, _I, _I, _VP, _I, _VP]),
     "tdvh_unit_float_to_u8": (_I, [_VP, _VP, _I, _I, _I, _VP, _I, _VP]),
     "tdvh_sgm_paths": (_SZ, [_I, _I, _VP]),
+    "tdvh_prefilter_gauss5": (_I, [_VP, _VP, _I, _I, _I, _VP, _I, _VP]),
+    "tdvh_postfilter_open": (_I, [_VP, _VP, _I, _I, _I, _I, _VP, _I, _VP]),
+    "tdvh_rgb_to_gray": (_I, [_VP, _VP, _I, _I, _I, _I, _I, _VP, _I, _VP]),
+    "tdvh_remap_u8": (_I, [_VP, _VP, _I, _I, _I, _VP, _VP, _I, _I, _I, _I, _VP, _I, _VP]),
+    "tdvh_reproject": (_I, [_VP, _VP, _I, _I, _I, _VP, ct.c_float, ct.c_float, _VP, _I, _VP]),
+    "tdvh_host_alloc": (_I, [_SZ, ct.POINTER(_VP)]),
+    "tdvh_host_free": (_I, [_VP]),
+    "tdvh_copy_2d": (_I, [_VP, _VP, _SZ, _VP, _SZ, _SZ, _I, _VP]),
+    "tdvh_dsi_download": (_I, [_VP, ct.POINTER(DSI), _VP, _VP]),
     "tdvh_batch_create": (_I, [_VP, ct.POINTER(Pair), _I, ct.POINTER(SgmParams),
                                ct.POINTER(_VP)]),
     "tdvh_batch_destroy": (_I, [_VP]),
@@ -336,6 +345,94 @@ def unit_float_to_u8(img, ctx=None, stream=None):
     _check(lib().tdvh_unit_float_to_u8(_ctx(ctx), _ptr(img), img.stride(0), W, H, _ptr(out), W,
                                        _stream(stream)))
     return out
+
+
+RGB, BGR = 0, 1
+
+
+def prefilter_gauss5(img, out=None, ctx=None, stream=None):
+    """5x5 binomial Gaussian of a uint8 (H, W) device image."""
+    H, W = img.shape
+    if out is None:
+        out = torch.empty((H, W), dtype=torch.uint8, device=img.device)
+    _check(lib().tdvh_prefilter_gauss5(_ctx(ctx), _ptr(img), img.stride(0), W, H, _ptr(out),
+                                       out.stride(0), _stream(stream)))
+    return out
+
+
+def postfilter_open(disp, iterations=3, out=None, ctx=None, stream=None):
+    """`iterations` 3x3 erosions then dilations of int16 (H, W) disparities."""
+    H, W = disp.shape
+    if out is None:
+        out = torch.empty((H, W), dtype=torch.int16, device=disp.device)
+    _check(lib().tdvh_postfilter_open(_ctx(ctx), _ptr(disp), disp.stride(0), W, H, iterations,
+                                      _ptr(out), out.stride(0), _stream(stream)))
+    return out
+
+
+def rgb_to_gray(img, order=RGB, ctx=None, stream=None):
+    """uint8 (H, W, 3 or 4) colour image on the device -> uint8 (H, W) gray."""
+    H, W, C = img.shape
+    assert img.stride(2) == 1 and img.stride(1) == C
+    out = torch.empty((H, W), dtype=torch.uint8, device=img.device)
+    _check(lib().tdvh_rgb_to_gray(_ctx(ctx), _ptr(img), img.stride(0), W, H, C, order, _ptr(out),
+                                  W, _stream(stream)))
+    return out
+
+
+def remap(img, map_x, map_y, border=0, ctx=None, stream=None):
+    """Bilinear remap of a uint8 (H, W) image by float (Ho, Wo) maps."""
+    H, W = img.shape
+    Ho, Wo = map_x.shape
+    assert map_y.shape == map_x.shape and map_x.stride(0) == map_y.stride(0)
+    out = torch.empty((Ho, Wo), dtype=torch.uint8, device=img.device)
+    _check(lib().tdvh_remap_u8(_ctx(ctx), _ptr(img), img.stride(0), W, H, _ptr(map_x),
+                               _ptr(map_y), map_x.stride(0), Wo, Ho, border, _ptr(out), Wo,
+                               _stream(stream)))
+    return out
+
+
+def reproject(disp, Q, disp_scale=1.0, z_scale=24.0, ctx=None, stream=None):
+    """int16 (H, W) disparities -> float32 (H, W, 3) points via the 4x4 Q."""
+    H, W = disp.shape
+    q = (ct.c_float * 16)(*[float(v) for v in list(Q.reshape(16))])
+    out = torch.empty((H, W, 3), dtype=torch.float32, device=disp.device)
+    _check(lib().tdvh_reproject(_ctx(ctx), _ptr(disp), disp.stride(0), W, H,
+                                ct.cast(q, ct.c_void_p), disp_scale, z_scale, _ptr(out), 3 * W,
+                                _stream(stream)))
+    return out
+
+
+class PinnedBuffer:
+    """Page-locked host bytes (tdvh_host_alloc) viewed as a numpy array."""
+
+    def __init__(self, shape, dtype):
+        import numpy as np
+        self.dtype = np.dtype(dtype)
+        n = int(np.prod(shape)) * self.dtype.itemsize
+        h = ct.c_void_p()
+        _check(lib().tdvh_host_alloc(n, ct.byref(h)))
+        self.ptr = h
+        raw = (ct.c_uint8 * n).from_address(h.value)
+        self.array = np.frombuffer(raw, dtype=self.dtype).reshape(shape)
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            _check(lib().tdvh_host_free(self.ptr))
+            self.ptr = None
+
+
+def copy_2d(dst_ptr, dst_pitch, src_ptr, src_pitch, width_bytes, rows, ctx=None, stream=None):
+    """One asynchronous 2D copy between any two buffers (pitches in bytes)."""
+    _check(lib().tdvh_copy_2d(_ctx(ctx), ct.c_void_p(dst_ptr), dst_pitch, ct.c_void_p(src_ptr),
+                              src_pitch, width_bytes, rows, _stream(stream)))
+
+
+def dsi_download(dsi, host_ptr, ctx=None, stream=None):
+    """Copies a dense DSI (a DSI struct) to host memory in one transfer."""
+    _check(lib().tdvh_dsi_download(_ctx(ctx), ct.byref(dsi), ct.c_void_p(host_ptr),
+                                   _stream(stream)))
 
 
 def sgm_paths(width, height):
