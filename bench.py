@@ -167,6 +167,96 @@ def bench_served(torch, duration=1.0, rate=600.0, frames_per_stream=4, slots=4):
     return out
 
 
+def bench_boundary(tv, torch, ctx, L, R, D, reps=20):
+    """Steps around the path at the headline shape (device-resident unless
+    stated): Gaussian prefilter, erode/dilate postfilter, colour to gray,
+    rectification remap, reprojection (per-kernel hipEvent times and HBM
+    rates), the graph pipeline vs the same stages called one by one, and the
+    PCIe-inclusive rate from pinned host images to pinned host disparities."""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    H, W = L.shape
+    Ld, Rd = torch.from_numpy(L).to(dev), torch.from_numpy(R).to(dev)
+    rgb = torch.from_numpy(np.stack([L, R, L], -1)).to(dev)
+    yy, xx = np.mgrid[0:H, 0:W].astype(np.float32)
+    mx = torch.from_numpy(0.995 * xx + 0.01 * yy + 2.0).to(dev)
+    my = torch.from_numpy(-0.01 * xx + 1.005 * yy - 1.0).to(dev)
+    disp = tv.stereo_bt_sgm(Ld, Rd, D, ctx=ctx)
+    Q = np.array([[1, 0, 0, -W / 2], [0, 1, 0, -H / 2], [0, 0, 0, 700.0], [0, 0, 8.0, 0]],
+                 np.float32)
+    px = W * H
+    steps = {
+        "gauss5_kernel": (lambda: tv.prefilter_gauss5(Ld, ctx=ctx), 2 * px),
+        "morph_kernel": (lambda: tv.postfilter_open(disp, 3, ctx=ctx), 8 * px),
+        "gray_kernel": (lambda: tv.rgb_to_gray(rgb, ctx=ctx), 4 * px),
+        "remap_kernel": (lambda: tv.remap(Ld, mx, my, ctx=ctx), 10 * px),
+        "reproject_kernel": (lambda: tv.reproject(disp, Q, 1.0 / D, 24.0, ctx=ctx), 14 * px),
+    }
+    out = {"workload": "%dx%d uint8 pair, D=%d" % (W, H, D), "kernels": {}}
+    for name, (fn, nbytes) in steps.items():
+        fn()
+        torch.cuda.synchronize()
+        ctx.profile(True)
+        ctx.profile_reset()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        t = ctx.profile_table()
+        ctx.profile(False)
+        ms = sum(v[0] for k, v in t.items() if k.startswith(name)) / reps
+        out["kernels"][name] = {"ms": round(ms, 4), "bytes": nbytes,
+                                "GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+
+    def timed(fn, n):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n * 1e3
+
+    pipe = tv.Pipeline(W, H, D, flags=tv.PIPE_PREFILTER | tv.PIPE_POSTFILTER)
+    res = torch.empty((H, W), dtype=torch.int16, device=dev)
+    ms_graph = timed(lambda: pipe.run(Ld, Rd, out=res), reps)
+    lf = torch.empty_like(Ld)
+    rf = torch.empty_like(Rd)
+
+    def eager():
+        tv.prefilter_gauss5(Ld, out=lf, ctx=ctx)
+        tv.prefilter_gauss5(Rd, out=rf, ctx=ctx)
+        tv.stereo_bt_sgm(lf, rf, D, out=res, ctx=ctx)
+        tv.postfilter_open(res, 3, out=res, ctx=ctx)
+
+    ms_eager = timed(eager, reps)
+    # PCIe-inclusive: pinned host images in, pinned host disparities out
+    hl, hr = tv.PinnedBuffer((H, W), np.uint8), tv.PinnedBuffer((H, W), np.uint8)
+    hd = tv.PinnedBuffer((H, W), np.int16)
+    hl.array[:], hr.array[:] = L, R
+    Lb, Rb = torch.empty_like(Ld), torch.empty_like(Rd)
+
+    def host_frame():
+        tv.copy_2d(Lb.data_ptr(), W, hl.array.ctypes.data, W, W, H)
+        tv.copy_2d(Rb.data_ptr(), W, hr.array.ctypes.data, W, W, H)
+        pipe.run(Lb, Rb, out=res)
+        tv.copy_2d(hd.array.ctypes.data, 2 * W, res.data_ptr(), 2 * W, 2 * W, H)
+
+    ms_host = timed(host_frame, reps)
+    exact = bool((hd.array == res.cpu().numpy()).all())
+    for b in (hl, hr, hd):
+        b.free()
+    pipe.close()
+    cells = W * H * D
+    out["pipeline"] = {"stages": "gauss5 x2 -> BT+SGM8+WTA -> erode/dilate x3",
+                       "graph_ms_per_frame": round(ms_graph, 4),
+                       "eager_ms_per_frame": round(ms_eager, 4),
+                       "graph_MDE/s": round(cells / (ms_graph * 1e-3) / 1e6, 1),
+                       "pcie_inclusive_ms_per_frame": round(ms_host, 4),
+                       "pcie_inclusive_MDE/s": round(cells / (ms_host * 1e-3) / 1e6, 1),
+                       "host_result_exact": exact}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,6 +349,8 @@ def main():
     if not args.no_served:
         served = bench_served(torch)
 
+    boundary = bench_boundary(tv, torch, ctx, L, R, D)
+
     cpu = None
     if not args.no_cpu_baseline:
         import oracle
@@ -306,6 +398,7 @@ def main():
         "batched": batched,
         "batched_720p": batched_hd,
         "served": served,
+        "boundary": boundary,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -285,6 +285,27 @@ int tdvh_copy_2d(tdvh_ctx *ctx, void *dst, size_t dst_pitch, const void *src,
  * one cudaMemcpy per (x, y). */
 int tdvh_dsi_download(tdvh_ctx *ctx, const tdvh_dsi *dsi, void *host, void *stream);
 
+/* ---- stream/graph executor for a fixed frame shape ----------------------------- */
+
+/* A device-resident stereo pipeline for one frame shape, replayed as one
+ * hipGraph per frame: [Gaussian prefilter of both inputs] -> fused BT + SGM-8
+ * + WTA -> [erode/dilate postfilter, 3 iterations]. Replaces the reference's
+ * execution of DevStereoMatcher's WorkUnits on one CUDAProcess thread with a
+ * blocking kernel sequence per stage (cudaprocess.cpp:9-30,
+ * devstereomatcher.cpp:14-49) and the Dilate stage of Reconstruction
+ * (reconstruction.cpp:30-31). tdvh_pipeline_run copies the images into the
+ * pipeline's buffers, launches the graph and copies the disparities out, all
+ * asynchronously on `stream`; the first run executes eagerly and captures the
+ * graph. The pipeline owns a private context; runs on one pipeline must not
+ * overlap (they share its buffers). */
+typedef struct tdvh_pipeline tdvh_pipeline;
+enum { TDVH_PIPE_PREFILTER = 1, TDVH_PIPE_POSTFILTER = 2 };
+int tdvh_pipeline_create(tdvh_ctx *ctx, int width, int height, int depth,
+                         const tdvh_sgm_params *params, int flags, tdvh_pipeline **out);
+int tdvh_pipeline_run(tdvh_pipeline *p, const uint8_t *left, const uint8_t *right, int pitch,
+                      int16_t *disp, int disp_pitch, void *stream);
+int tdvh_pipeline_destroy(tdvh_pipeline *p);
+
 typedef struct tdvh_batch tdvh_batch;
 
 /* Plans a batch (work queue, scratch) for `n` pairs. The pair descriptors

@@ -33,6 +33,29 @@ private:
     ReadWritePipe<FloatImage> m_wpipe;
 };
 
+// Frame-pair gate in front of a matcher (CtrlWork, ctrlprocess.cpp:27-73):
+// passes or drops left/right pairs according to the FlowCtrl mode.
+class CtrlWork : public WorkUnit, public FlowCtrl {
+public:
+    CtrlWork() { workName("Flow control"); }
+    void inputs(ReadPipe<FloatImage> *l, ReadPipe<FloatImage> *r)
+    {
+        m_lrpipe = l;
+        m_rrpipe = r;
+    }
+    ReadPipe<FloatImage> *leftImgOutput() { return &m_lwpipe; }
+    ReadPipe<FloatImage> *rightImgOutput() { return &m_rwpipe; }
+    bool update();
+    size_t passed() const { return m_passed; }
+    size_t dropped() const { return m_dropped; }
+
+private:
+    ReadPipe<FloatImage> *m_lrpipe = nullptr, *m_rrpipe = nullptr;
+    ReadWritePipe<FloatImage> m_lwpipe, m_rwpipe;
+    std::atomic<size_t> m_passed{0}, m_dropped{0};
+};
+typedef WorkUnitProcess<CtrlWork> CtrlProcess;
+
 // Gaussian 5x5 prefilter of an input image on the device (the reference's
 // "median filter" stage: MedianFilterCPU, medianfiltercpu.cpp:7-19, which
 // DevStereoMatcher uses; MedianFilterDev, medianfilterdev.cpp:11-27, is the
@@ -127,6 +150,29 @@ private:
     ReadWritePipe<FloatImage> m_wpipe;
     Benchmark m_mark;
     Process *m_procs[2];
+};
+
+// DevStereoMatcher's BT + SGM chain (with its Gaussian prefilter and,
+// optionally, the Dilate postfilter) as one hipGraph per frame
+// (tdvh_pipeline): the stages run as graph nodes on the device instead of
+// WorkUnits with blocking kernel sequences. The pipeline is built for the
+// first frame's shape and rebuilt when the shape changes.
+class GraphStereoMatcher : public FusedMatcher {
+public:
+    explicit GraphStereoMatcher(int maxDisparity, bool postfilter = false, int device = 0)
+        : FusedMatcher(maxDisparity, device), m_postfilter(postfilter)
+    {
+    }
+    ~GraphStereoMatcher();
+    std::string name() const { return "GraphStereoMatcher"; }
+
+protected:
+    void run(FloatImage left, FloatImage right, FloatImage out);
+
+private:
+    bool m_postfilter;
+    tdvh_pipeline *m_pipe = nullptr;
+    int m_w = 0, m_h = 0;
 };
 
 // SSD 7x7 + lowest-index WTA without a cost volume.

@@ -50,6 +50,61 @@ private:
 };
 typedef DeviceProcess CUDAProcess;
 
+// Runs one work unit on its own thread until it returns false (the
+// reference's TWorkUnitProcess, workunitprocess.hpp).
+template <typename Unit>
+class WorkUnitProcess : public Unit, public Process {
+public:
+    void process()
+    {
+        while (!m_end && this->update()) {
+        }
+    }
+    void finish() { m_end = true; }
+
+private:
+    std::atomic<bool> m_end{false};
+};
+
+// Flow control of a frame source (ctrlprocess.hpp:13-57, ctrlprocess.cpp:6-24):
+// in Step mode one frame passes per step(); in Continuous mode every frame
+// passes while the consumer's input pipes have room, and frames arriving while
+// they are full are dropped.
+class FlowCtrl {
+public:
+    enum Mode { Continuous, Step };
+    void pause() { m_step = false; }
+    void step()
+    {
+        m_mode = Step;
+        m_step = true;
+    }
+    void stepMode()
+    {
+        m_mode = Step;
+        m_step = false;
+    }
+    void continuous()
+    {
+        m_step = true;
+        m_mode = Continuous;
+    }
+    Mode mode() const { return m_mode; }
+
+protected:
+    // Whether the next frame may pass (consumes a pending step).
+    bool testFlow()
+    {
+        if (!m_step) return false;
+        if (m_mode == Step) m_step = false;
+        return true;
+    }
+
+private:
+    std::atomic<bool> m_step{false};
+    std::atomic<Mode> m_mode{Step};
+};
+
 // Receives exceptions escaping a process thread (exceptionreport.hpp).
 class ExceptionReport {
 public:

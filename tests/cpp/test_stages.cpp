@@ -240,6 +240,33 @@ static void test_gpu(const std::string &dir)
         for (size_t i = 0; i < d.size(); i++) ok = ok && fv[i] == (float)d[i] / (float)c.D;
         CHECK(ok);
     }
+    { // the chain as one graph per frame (prefilter [+ Dilate]), two frames each
+        for (int post = 0; post < 2; post++) {
+            GraphStereoMatcher m(c.D, post != 0);
+            ReadWritePipe<FloatImage> lp(2), rp(2);
+            m.inputs(&lp, &rp);
+            ProcessRunner runner(m);
+            runner.run();
+            for (int k = 0; k < 2; k++) {
+                lp.write(FloatImage::FromGray8((const uint8_t *)c.L.data(), c.W, c.H));
+                rp.write(FloatImage::FromGray8((const uint8_t *)c.R.data(), c.W, c.H));
+            }
+            lp.finish();
+            rp.finish();
+            const auto want = slurp(dir + (post ? "/bt_sgm_pf_open.i16" : "/bt_sgm_pf.i16"));
+            for (int k = 0; k < 2; k++) {
+                FloatImage out;
+                CHECK(m.output()->read(&out));
+                std::vector<int16_t> d(c.W * c.H);
+                checkHip(hipMemcpy(d.data(), out.disparityDev(), d.size() * 2,
+                                   hipMemcpyDeviceToHost),
+                         "copy disparity");
+                CHECK(same(d, want, post ? "graph chain + Dilate" : "graph chain"));
+            }
+            runner.join();
+            CHECK(!runner.errorOcurred());
+        }
+    }
     { // fused BT inside SGM
         SemiglobalLabMatcher m(c.D);
         CHECK(same(run_matcher(m, c, nullptr), slurp(dir + "/bt_sgm.i16"), "SemiglobalLab"));
@@ -301,6 +328,56 @@ static void test_gpu(const std::string &dir)
     }
 }
 
+// CtrlWork (ctrlprocess.cpp:27-73): step mode passes one pair per step();
+// continuous mode passes pairs while the outputs have room, drops the rest.
+static void test_flow_control()
+{
+    auto frame = [] { return FloatImage::CreateCPU(Dim(2, 2)); };
+    {
+        ReadWritePipe<FloatImage> lin(8), rin(8);
+        CtrlWork cw;
+        cw.inputs(&lin, &rin);
+        for (int i = 0; i < 3; i++) {
+            lin.write(frame());
+            rin.write(frame());
+        }
+        CHECK(cw.mode() == FlowCtrl::Step);
+        CHECK(cw.update()); // paused: dropped
+        CHECK(cw.passed() == 0 && cw.dropped() == 1);
+        cw.step();
+        CHECK(cw.update()); // one step: passes
+        CHECK(cw.passed() == 1);
+        CHECK(cw.update()); // the step is consumed: dropped
+        CHECK(cw.passed() == 1 && cw.dropped() == 2);
+        FloatImage a, b;
+        CHECK(cw.leftImgOutput()->read(&a) && cw.rightImgOutput()->read(&b));
+    }
+    {
+        ReadWritePipe<FloatImage> lin(8), rin(8);
+        CtrlWork cw;
+        cw.inputs(&lin, &rin);
+        cw.continuous();
+        for (int i = 0; i < 4; i++) {
+            lin.write(frame());
+            rin.write(frame());
+        }
+        lin.finish();
+        rin.finish();
+        CHECK(cw.update()); // outputs empty: passes
+        CHECK(cw.update()); // outputs full (capacity 1, not read): dropped
+        CHECK(cw.passed() == 1 && cw.dropped() == 1);
+        FloatImage a, b;
+        CHECK(cw.leftImgOutput()->read(&a) && cw.rightImgOutput()->read(&b));
+        CHECK(cw.update()); // room again: passes
+        CHECK(cw.passed() == 2);
+        CHECK(cw.leftImgOutput()->read(&a) && cw.rightImgOutput()->read(&b));
+        CHECK(cw.update()); // the 4th pair passes
+        CHECK(!cw.update()); // inputs finished: outputs finish
+        CHECK(cw.leftImgOutput()->read(&a)); // the 4th pair is still readable
+        CHECK(!cw.leftImgOutput()->read(&a));
+    }
+}
+
 int main(int argc, char **argv)
 {
     const std::string mode = argc > 1 ? argv[1] : "cpu";
@@ -309,6 +386,7 @@ int main(int argc, char **argv)
         test_bounded_buffer();
         test_pipe_and_filter();
         test_runner_reports_errors();
+        test_flow_control();
     } else if (mode == "gpu" && argc > 2) {
         try {
             test_gpu(argv[2]);

@@ -1,0 +1,56 @@
+"""The fixed-shape graph pipeline (tdvh_pipeline_*): every replay equals the
+oracle's prefilter -> BT + SGM -> WTA -> postfilter on the same pair, with
+eager library work and synchronisations between replays."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from thundervision_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def want(L, R, D, flags):
+    if flags & 1:
+        L, R = oracle.gauss5(L), oracle.gauss5(R)
+    d = oracle.stereo_bt_sgm(L, R, D)
+    if flags & 2:
+        d = oracle.open_disp(d, 3)
+    return d
+
+
+@pytest.mark.parametrize("flags", [0, 1, 2, 3])
+def test_pipeline_replays(tv, dev, flags):
+    W, H, D = 200, 120, 64
+    p = tv.Pipeline(W, H, D, flags=flags)
+    try:
+        for k in range(4):
+            L, R, _ = synth.pair(W, H, D - 1, 6, 40 + k)
+            Ld, Rd = torch.from_numpy(L).to(dev), torch.from_numpy(R).to(dev)
+            got = p.run(Ld, Rd)
+            # eager work of the library on another context between replays
+            tv.stereo_bt_sgm(Rd, Ld, 32)
+            torch.cuda.synchronize()
+            assert (got.cpu().numpy() == want(L, R, D, flags)).all(), k
+    finally:
+        p.close()
+
+
+def test_pipeline_720p_pitched(tv, dev):
+    L, R, _ = synth.workload_pair(synth.PLAIN_720P)
+    H, W = L.shape
+    p = tv.Pipeline(W, H, 128, flags=3)
+    try:
+        big = torch.zeros((2, H, W + 64), dtype=torch.uint8, device=dev)
+        big[0, :, :W] = torch.from_numpy(L).to(dev)
+        big[1, :, :W] = torch.from_numpy(R).to(dev)
+        w = want(L, R, 128, 3)
+        for _ in range(3):
+            out = torch.full((H, W + 8), -7, dtype=torch.int16, device=dev)
+            p.run(big[0, :, :W], big[1, :, :W], out=out[:, :W])
+            torch.cuda.synchronize()
+            assert (out[:, :W].cpu().numpy() == w).all()
+            assert (out[:, W:] == -7).all()
+    finally:
+        p.close()

@@ -17,7 +17,7 @@ BUILD = os.path.join(PKG, "_build")
 INCLUDE = os.path.join(ROOT, "include")
 
 HIP_SOURCES = ["ctx.hip", "cost_bt.hip", "cost_window.hip", "dp.hip", "sgm.hip", "sgm_ref.hip",
-               "wta.hip", "batch.hip", "io.hip"]
+               "wta.hip", "batch.hip", "io.hip", "pipeline.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -22,6 +22,25 @@ DevStereoMatcher::DevStereoMatcher(int device) : m_process(device)
     m_procs[1] = nullptr;
 }
 
+bool CtrlWork::update()
+{
+    FloatImage l, r;
+    if (!(m_lrpipe->read(&l) && m_rrpipe->read(&r))) {
+        m_lwpipe.finish();
+        m_rwpipe.finish();
+        return false;
+    }
+    if (testFlow() &&
+        (mode() != Continuous || (!m_lwpipe.isFull() && !m_rwpipe.isFull()))) {
+        m_lwpipe.write(l);
+        m_rwpipe.write(r);
+        m_passed++;
+    } else {
+        m_dropped++;
+    }
+    return true;
+}
+
 bool MedianFilterDev::update()
 {
     WriteGuard<ReadWritePipe<FloatImage>> guard(m_wpipe);
@@ -130,6 +149,30 @@ void SemiglobalLabMatcher::run(FloatImage left, FloatImage right, FloatImage out
                                  W, m_maxDisparity, &p, TDVH_FUSE_COST, out.disparityDev(), W,
                                  nullptr),
               "tdvh_stereo_bt_sgm");
+}
+
+GraphStereoMatcher::~GraphStereoMatcher()
+{
+    if (m_pipe) (void)tdvh_pipeline_destroy(m_pipe);
+}
+
+void GraphStereoMatcher::run(FloatImage left, FloatImage right, FloatImage out)
+{
+    const int W = (int)left.dim().width(), H = (int)left.dim().height();
+    if (!m_pipe || W != m_w || H != m_h) {
+        if (m_pipe) checkTdvh(tdvh_pipeline_destroy(m_pipe), "tdvh_pipeline_destroy");
+        m_pipe = nullptr;
+        const tdvh_sgm_params p{8, 32, 0xFF, 0};
+        const int flags = TDVH_PIPE_PREFILTER | (m_postfilter ? TDVH_PIPE_POSTFILTER : 0);
+        checkTdvh(tdvh_pipeline_create(deviceContext(m_device), W, H, m_maxDisparity, &p, flags,
+                                       &m_pipe),
+                  "tdvh_pipeline_create");
+        m_w = W;
+        m_h = H;
+    }
+    checkTdvh(tdvh_pipeline_run(m_pipe, left.devMemU8(), right.devMemU8(), W, out.disparityDev(),
+                                W, nullptr),
+              "tdvh_pipeline_run");
 }
 
 StereoMatcher *CommonStereoMatcherFactory::createStereoMatcher()

@@ -90,6 +90,9 @@ SIGNATURES = {
     "tdvh_host_free": (_I, [_VP]),
     "tdvh_copy_2d": (_I, [_VP, _VP, _SZ, _VP, _SZ, _SZ, _I, _VP]),
     "tdvh_dsi_download": (_I, [_VP, ct.POINTER(DSI), _VP, _VP]),
+    "tdvh_pipeline_create": (_I, [_VP, _I, _I, _I, ct.POINTER(SgmParams), _I, ct.POINTER(_VP)]),
+    "tdvh_pipeline_run": (_I, [_VP, _VP, _VP, _I, _VP, _I, _VP]),
+    "tdvh_pipeline_destroy": (_I, [_VP]),
     "tdvh_batch_create": (_I, [_VP, ct.POINTER(Pair), _I, ct.POINTER(SgmParams),
                                ct.POINTER(_VP)]),
     "tdvh_batch_destroy": (_I, [_VP]),
@@ -433,6 +436,34 @@ def dsi_download(dsi, host_ptr, ctx=None, stream=None):
     """Copies a dense DSI (a DSI struct) to host memory in one transfer."""
     _check(lib().tdvh_dsi_download(_ctx(ctx), ct.byref(dsi), ct.c_void_p(host_ptr),
                                    _stream(stream)))
+
+
+PIPE_PREFILTER, PIPE_POSTFILTER = 1, 2
+
+
+class Pipeline:
+    """Fixed-shape stereo pipeline replayed as a hipGraph (tdvh_pipeline_*)."""
+
+    def __init__(self, width, height, depth, p1=8, p2=32, flags=PIPE_PREFILTER, ctx=None):
+        self.width, self.height, self.depth = width, height, depth
+        params = SgmParams(p1, p2, 0xFF, 0)
+        h = ct.c_void_p()
+        _check(lib().tdvh_pipeline_create(_ctx(ctx), width, height, depth, ct.byref(params),
+                                          flags, ct.byref(h)))
+        self.handle = h
+
+    def run(self, left, right, out=None, stream=None):
+        if out is None:
+            out = torch.empty((self.height, self.width), dtype=torch.int16, device=left.device)
+        assert left.stride(0) == right.stride(0)
+        _check(lib().tdvh_pipeline_run(self.handle, _ptr(left), _ptr(right), left.stride(0),
+                                       _ptr(out), out.stride(0), _stream(stream)))
+        return out
+
+    def close(self):
+        if self.handle:
+            _check(lib().tdvh_pipeline_destroy(self.handle))
+            self.handle = None
 
 
 def sgm_paths(width, height):
