@@ -295,19 +295,26 @@ __device__ __forceinline__ void buf_load(buf_t r, uint32_t off, uint32_t *dst)
     }
 }
 
+// Cache-policy operand of the plane stores: non-temporal (the planes are
+// read once, by a later kernel; measured: the sum over them runs ~10 %
+// faster than after ordinary write-back stores).
+constexpr int BUF_STORE_NT = 2;
+
 // N consecutive dwords to byte offset off (dropped outside the buffer).
 template <int N>
 __device__ __forceinline__ void buf_store(buf_t r, uint32_t off, const uint32_t *src)
 {
     if constexpr (N == 4) {
         __builtin_amdgcn_raw_buffer_store_b128(tdvh_u32x4{src[0], src[1], src[2], src[3]}, r,
-                                               off, 0, 0);
+                                               off, 0, BUF_STORE_NT);
     } else if constexpr (N == 3) {
-        __builtin_amdgcn_raw_buffer_store_b96(tdvh_u32x3{src[0], src[1], src[2]}, r, off, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b96(tdvh_u32x3{src[0], src[1], src[2]}, r, off, 0,
+                                              BUF_STORE_NT);
     } else if constexpr (N == 2) {
-        __builtin_amdgcn_raw_buffer_store_b64(tdvh_u32x2{src[0], src[1]}, r, off, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b64(tdvh_u32x2{src[0], src[1]}, r, off, 0,
+                                              BUF_STORE_NT);
     } else if constexpr (N == 1) {
-        __builtin_amdgcn_raw_buffer_store_b32(src[0], r, off, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(src[0], r, off, 0, BUF_STORE_NT);
     }
 }
 
@@ -430,7 +437,7 @@ __device__ __forceinline__ void sgm_line_group_buf(const SgmArgs &a, int dir, in
 #pragma unroll
             for (int r = 0; r < M; r++)
                 __builtin_amdgcn_raw_buffer_store_b16((unsigned short)pack_e(E[r], 0u), out,
-                                                      so + 2 * r, 0, 0);
+                                                      so + 2 * r, 0, BUF_STORE_NT);
         }
         costs(ring[(u + 1) % PF], c);
         // refill only after the slot's words are consumed, so the loads
