@@ -133,17 +133,6 @@ __device__ __forceinline__ uint32_t pk_add(uint32_t a, uint32_t b) { return u32(
 __device__ __forceinline__ uint32_t pk_sub(uint32_t a, uint32_t b) { return u32(v2(a) - v2(b)); }
 __device__ __forceinline__ uint32_t pk_splat(uint32_t v) { return (v & 0xFFFFu) * 0x00010001u; }
 
-// DPP moves inside a 16-lane row (old = fill for lanes without a source).
-// row_ror:1 -> lane i reads lane (i-1) mod 16; row_ror:15 -> lane (i+1) mod 16.
-__device__ __forceinline__ uint32_t row_ror1(uint32_t v)
-{
-    return (uint32_t)__builtin_amdgcn_update_dpp((int)0xFFFFFFFF, (int)v, 0x121, 0xF, 0xF, false);
-}
-__device__ __forceinline__ uint32_t row_ror15(uint32_t v)
-{
-    return (uint32_t)__builtin_amdgcn_update_dpp((int)0xFFFFFFFF, (int)v, 0x12F, 0xF, 0xF, false);
-}
-
 // Minimum over the 16 lanes of a DPP row; every lane receives it.
 __device__ __forceinline__ uint32_t row_min_u32(uint32_t v)
 {

@@ -106,53 +106,109 @@ static LineSet make_lineset(int W, int H, int dir_mask)
     return ls;
 }
 
+// Spatial coordinate of a line across its direction: lines with close keys
+// read neighbouring rows/columns/diagonals of the term planes.
+static int line_key(int dir, int sx, int sy)
+{
+    switch (dir) {
+    case 0: case 1: return sy;            // horizontal: row
+    case 2: case 3: return sx;            // vertical: column
+    case 4: case 5: return sx - sy;       // (1,1) / (-1,-1) diagonals
+    default: return sx + sy;              // (-1,1) / (1,-1) anti-diagonals
+    }
+}
+
+// Chiplets (XCDs) of the device, each with its own L2; workgroup b of a
+// launch runs on chiplet b % NXCD.
+constexpr int NXCD = 8;
+
 // Device table of the line groups (4 consecutive lines of one direction)
-// of every enabled direction, longest first: with one wave per workgroup the
-// dispatcher then runs a longest-processing-time-first list schedule.
-// Entry: plane slot << 28 | direction << 24 | first line. Built and uploaded
-// once per (W, H, dir_mask) and context.
+// of every enabled direction. Entry: plane slot << 28 | direction << 24 |
+// first line; workgroup b runs entry b. Two goals:
+//  * longest first: with one wave per workgroup the dispatcher then runs a
+//    longest-processing-time-first list schedule;
+//  * chiplet locality: each direction's groups, in spatial order, are cut
+//    into NXCD bands of equal work and band c goes to chiplet c, so the
+//    waves sharing rows of the term planes share one L2 (round-robin
+//    placement would make every chiplet fetch every row).
+// Entry b is the next entry of chiplet b % NXCD's longest-first queue.
+// Built and uploaded once per (W, H, dir_mask) and context.
 static int line_order(tdvh_ctx *ctx, int W, int H, int dir_mask, const uint32_t **out,
                       int *groups)
 {
     TDVH_REQUIRE(W + H < (1 << 24), "image %dx%d too large for the line table", W, H);
-    std::vector<std::pair<int, uint32_t>> g;
+    struct Group {
+        int len, key;
+        uint32_t entry;
+    };
+    std::vector<std::pair<int, uint32_t>> q[NXCD]; // (length, entry) per chiplet
+    size_t total = 0;
     int slot = 0;
     for (int dir = 0; dir < 8; dir++) {
         if (!(dir_mask & (1 << dir))) continue;
         const int n = dir_lines(dir, W, H);
+        std::vector<Group> g;
+        long work = 0;
         for (int l0 = 0; l0 < n; l0 += 4) {
-            int kmax = 0;
+            int kmax = 0, key = 0;
             for (int l = l0; l < l0 + 4 && l < n; l++) {
                 int sx, sy, len;
                 line_geom(dir, l, W, H, sx, sy, len);
                 kmax = len > kmax ? len : kmax;
+                if (l == l0) key = line_key(dir, sx, sy);
             }
-            g.push_back({kmax, (uint32_t)slot << 28 | (uint32_t)dir << 24 | (uint32_t)l0});
+            g.push_back({kmax, key, (uint32_t)slot << 28 | (uint32_t)dir << 24 | (uint32_t)l0});
+            work += kmax;
         }
+        std::stable_sort(g.begin(), g.end(),
+                         [](const Group &x, const Group &y) { return x.key < y.key; });
+        long acc = 0;
+        for (const Group &e : g) {
+            const long mid = acc + e.len / 2; // band of the group's midpoint
+            const int c = (int)std::min<long>(NXCD - 1, mid * NXCD / std::max(work, 1L));
+            q[c].push_back({e.len, e.entry});
+            acc += e.len;
+        }
+        total += g.size();
         slot++;
     }
-    std::stable_sort(g.begin(), g.end(),
-                     [](const std::pair<int, uint32_t> &x, const std::pair<int, uint32_t> &y) {
-                         return x.first > y.first;
-                     });
+    for (auto &v : q)
+        std::stable_sort(v.begin(), v.end(),
+                         [](const std::pair<int, uint32_t> &x, const std::pair<int, uint32_t> &y) {
+                             return x.first > y.first;
+                         });
+    std::vector<uint32_t> h;
+    h.reserve(total);
+    size_t head[NXCD] = {};
+    for (size_t b = 0; b < total; b++) {
+        int c = (int)(b % NXCD);
+        if (head[c] == q[c].size()) {
+            // this chiplet's queue is drained: take the longest remaining head
+            int best = -1;
+            for (int o = 0; o < NXCD; o++)
+                if (head[o] < q[o].size() &&
+                    (best < 0 || q[o][head[o]].first > q[best][head[best]].first))
+                    best = o;
+            c = best;
+        }
+        h.push_back(q[c][head[c]++].second);
+    }
     // one table per shape and mask (a context serving several shapes keeps
     // them all, so steady-state calls never upload or synchronise)
     char name[64];
     snprintf(name, sizeof name, "sgm_line_order:%dx%d:%02x", W, H, dir_mask);
     void *buf = nullptr;
-    TDVH_TRY(ctx->scratch(name, g.size() * 4, &buf));
+    TDVH_TRY(ctx->scratch(name, h.size() * 4, &buf));
     tdvh_ctx::Slot &sl = ctx->slot(name);
     const uint64_t tag = ((uint64_t)W << 32) | ((uint64_t)H << 8) | (uint64_t)dir_mask;
     if (sl.tag != tag) {
-        std::vector<uint32_t> h(g.size());
-        for (size_t i = 0; i < g.size(); i++) h[i] = g[i].second;
         // queued launches may still read the previous table
         TDVH_CHECK_HIP(hipDeviceSynchronize());
         TDVH_CHECK_HIP(hipMemcpy(buf, h.data(), h.size() * 4, hipMemcpyHostToDevice));
         sl.tag = tag;
     }
     *out = (const uint32_t *)buf;
-    *groups = (int)g.size();
+    *groups = (int)h.size();
     return TDVH_OK;
 }
 

@@ -93,23 +93,34 @@ __device__ __forceinline__ void bt_costs(const uint32_t (&lt)[3], const uint32_t
 // One SGM step on the packed vector of a line. Lp: previous L (in/out),
 // c: costs of the current pixel, m: previous minimum (in/out); E receives
 // L - C = min(...) - m, in [0, P2] for valid cells.
+// Byte selectors of the neighbour fix-ups at the ends of a 16-lane row
+// (v_perm_b32: selector bytes 0-3 pick bytes of the operand, 13 gives 0xFF).
+// Lane 0: d-1 of d=0 is absent and d-1 of d=16m is lane 15's low half, i.e.
+// (A << 16) | 0xFFFF; lane 15: d+1 of d=16m-1 is lane 0's high half and d+1
+// of the last padded d is absent, i.e. (B >> 16) | 0xFFFF0000; other lanes
+// keep the rotated word.
+__device__ __forceinline__ uint32_t nb_sel_lo(int j) { return j == 0 ? 0x01000D0Du : 0x03020100u; }
+__device__ __forceinline__ uint32_t nb_sel_hi(int j) { return j == 15 ? 0x0D0D0302u : 0x03020100u; }
+
 template <int M>
 __device__ __forceinline__ void sgm_step(uint32_t (&Lp)[M], const uint32_t (&c)[M], uint32_t &m,
                                          int j, uint32_t p1, uint32_t P2, uint32_t (&E)[M])
 {
-    uint32_t A = row_ror1(Lp[M - 1]);
-    if (j == 0) A = (A << 16) | 0xFFFFu; // d-1 of d=0 is absent; of d=16m is lane 15's low
-    uint32_t B = row_ror15(Lp[0]);
-    if (j == 15) B = (B >> 16) | 0xFFFF0000u; // d+1 of d=16m-1 is lane 0's high; of Dp-1 absent
-    const uint32_t mm = pk_splat(m), mp2 = pk_splat(m + P2);
+    // row rotations write every lane of the row (no "old" operand), then one
+    // per-lane byte permute applies the end-of-row fix-ups
+    uint32_t A = (uint32_t)__builtin_amdgcn_mov_dpp((int)Lp[M - 1], 0x121, 0xF, 0xF, false);
+    A = __builtin_amdgcn_perm(A, A, nb_sel_lo(j));
+    uint32_t B = (uint32_t)__builtin_amdgcn_mov_dpp((int)Lp[0], 0x12F, 0xF, 0xF, false);
+    B = __builtin_amdgcn_perm(B, B, nb_sel_hi(j));
+    // every L'(d) >= m, so t >= m and min(t, m + P2) - m = min(t - m, P2)
+    const uint32_t mm = pk_splat(m), p2s = pk_splat(P2);
     uint32_t nmin = 0xFFFFFFFFu;
 #pragma unroll
     for (int r = 0; r < M; r++) {
         const uint32_t dm1 = r == 0 ? A : Lp[r - 1];
         const uint32_t dp1 = r == M - 1 ? B : Lp[r + 1];
-        uint32_t t = pk_adds(pk_min(dm1, dp1), p1);
-        t = pk_min(pk_min(t, Lp[r]), mp2);
-        E[r] = pk_sub(t, mm);
+        const uint32_t t = pk_min(pk_adds(pk_min(dm1, dp1), p1), Lp[r]);
+        E[r] = pk_min(pk_sub(t, mm), p2s);
     }
 #pragma unroll
     for (int r = 0; r < M; r++) {
