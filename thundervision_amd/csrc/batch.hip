@@ -194,7 +194,7 @@ extern "C" int tdvh_batch_create(tdvh_ctx *ctx, const tdvh_pair *pairs, int n,
         const int m = split_m(p.depth), Wr = p.width + 16 * m;
         const size_t W = p.width, H = p.height;
         off_terms[i] = total;
-        total = align(total + (3 * W * H + 3 * (size_t)Wr * H) * 4);
+        total = align(total + (W * H + 3 * (size_t)Wr * H) * 4);
         off_planes[i] = total;
         total = align(total + 8 * W * H * 32 * m);
         cells += (int64_t)W * H * p.depth;
@@ -317,7 +317,7 @@ extern "C" int tdvh_batch_create(tdvh_ctx *ctx, const tdvh_pair *pairs, int n,
         q.a.C = nullptr;
         q.a.Wr = p.width + 16 * m;
         q.a.LT = (const uint32_t *)(base + off_terms[i]);
-        q.a.RT = q.a.LT + 3 * (size_t)p.width * p.height;
+        q.a.RT = q.a.LT + (size_t)p.width * p.height;
         q.a.planes = base + off_planes[i];
         q.a.plane_bytes = (size_t)p.width * p.height * 32 * m;
         q.a.trash = base + trash_off;

@@ -358,10 +358,10 @@ static int sgm_std(tdvh_ctx *ctx, const uint32_t *Csplit, const uint8_t *left,
     a.trash = (uint8_t *)trash;
     if (!Csplit) {
         void *tb = nullptr;
-        const size_t words = 3 * (size_t)W * H + 3 * (size_t)a.Wr * H;
+        const size_t words = (size_t)W * H + 3 * (size_t)a.Wr * H;
         TDVH_TRY(ctx->scratch("bt_planes", words * 4, &tb));
         a.LT = (const uint32_t *)tb;
-        a.RT = (const uint32_t *)tb + 3 * (size_t)W * H;
+        a.RT = (const uint32_t *)tb + (size_t)W * H;
         TDVH_TRY(launch_bt_planes(left, right, W, H, pitch, D, (uint32_t *)a.LT,
                                   (uint32_t *)a.RT, s));
         if (!eout) {

@@ -46,7 +46,7 @@ __host__ __device__ inline void line_geom(int dir, int line, int W, int H, int &
 struct SgmArgs {
     int W, H, D, P1, P2;
     const uint32_t *C;  // SPLIT16 cost volume (staged), or null
-    const uint32_t *LT; // 3 planes W*H: splatted left BT terms l2, lmin, lmax (fused)
+    const uint32_t *LT; // W*H packed left BT terms (lt_unpack) (fused)
     const uint32_t *RT; // 3 planes H*Wr: paired right BT terms r2, rmin, rmax (fused)
     int Wr;             // row length of RT: W + 16m
     uint8_t *planes;    // path-cost planes (E bytes or L words)
@@ -70,6 +70,17 @@ __device__ __forceinline__ uint32_t cell_mask(int j, int r, int D, int x)
 {
     const int dlo = j * M + r, dhi = 16 * M + j * M + r;
     return ((dlo >= D || dlo > x) ? 0x0000FFFFu : 0u) | ((dhi >= D || dhi > x) ? 0xFFFF0000u : 0u);
+}
+
+// Splatted left BT terms of a pixel from its packed word (byte 0: L,
+// byte 1: 2L - lmin, byte 2: lmax - 2L; both differences lie in [0, 255]):
+// l2 = 2L, lmin and lmax in both halves.
+__device__ __forceinline__ void lt_unpack(uint32_t w, uint32_t &l2, uint32_t &lmn, uint32_t &lmx)
+{
+    const uint32_t l = __builtin_amdgcn_perm(w, w, 0x0C000C00u); // L | L << 16
+    l2 = pk_add(l, l);
+    lmn = pk_sub(l2, __builtin_amdgcn_perm(w, w, 0x0C010C01u));
+    lmx = pk_add(l2, __builtin_amdgcn_perm(w, w, 0x0C020C02u));
 }
 
 // BT2 costs of the 2M cells a lane owns at pixel (x, y). lt: splatted left
@@ -143,7 +154,7 @@ __device__ __forceinline__ void sgm_line_group(const SgmArgs &a, int dir, int li
                                                int prio)
 {
     constexpr int WPP = 16 * M;
-    constexpr int NW = FUSED ? 3 + 3 * M : M; // prefetched words per step
+    constexpr int NW = FUSED ? 1 + 3 * M : M; // prefetched words per step
     const int lane = threadIdx.x & 63;
     const int line = line0 + (lane >> 4);
     const int j = lane & 15;
@@ -160,7 +171,7 @@ __device__ __forceinline__ void sgm_line_group(const SgmArgs &a, int dir, int li
     const int kmax = max(max(__builtin_amdgcn_readlane(len, 0), __builtin_amdgcn_readlane(len, 16)),
                          max(__builtin_amdgcn_readlane(len, 32), __builtin_amdgcn_readlane(len, 48)));
     uint8_t *outp = a.planes + (size_t)s * a.plane_bytes;
-    const size_t lt_plane = (size_t)W * H, rt_plane = (size_t)a.Wr * H;
+    const size_t rt_plane = (size_t)a.Wr * H;
     // Long lines set the length of the launch: let their waves issue first
     // on a shared SIMD (instruction-issue priority within the CU only).
     if (prio) {
@@ -181,13 +192,12 @@ __device__ __forceinline__ void sgm_line_group(const SgmArgs &a, int dir, int li
             for (int r = 0; r < M; r++) slot[r] = src[r];
         } else {
             const int x = sx + kk * dx, y = sy + kk * dy;
-#pragma unroll
-            for (int t = 0; t < 3; t++) slot[t] = a.LT[t * lt_plane + p];
+            slot[0] = a.LT[p];
             const uint32_t *rt = a.RT + (size_t)y * a.Wr + (W - 1 - x) + j * M;
 #pragma unroll
             for (int t = 0; t < 3; t++)
 #pragma unroll
-                for (int r = 0; r < M; r++) slot[3 + t * M + r] = rt[t * rt_plane + r];
+                for (int r = 0; r < M; r++) slot[1 + t * M + r] = rt[t * rt_plane + r];
         }
     };
 
@@ -217,10 +227,11 @@ __device__ __forceinline__ void sgm_line_group(const SgmArgs &a, int dir, int li
             for (int r = 0; r < M; r++) c[r] = slot[r] | padm[r];
         } else {
             const int x = sx + kc * dx;
+            uint32_t l2, lmn, lmx;
+            lt_unpack(slot[0], l2, lmn, lmx);
 #pragma unroll
             for (int r = 0; r < M; r++) {
-                const uint32_t l2 = slot[0], lmn = slot[1], lmx = slot[2];
-                const uint32_t r2 = slot[3 + r], rmn = slot[3 + M + r], rmx = slot[3 + 2 * M + r];
+                const uint32_t r2 = slot[1 + r], rmn = slot[1 + M + r], rmx = slot[1 + 2 * M + r];
                 const uint32_t aa = pk_max(pk_subs(l2, rmx), pk_subs(rmn, l2));
                 const uint32_t bb = pk_max(pk_subs(r2, lmx), pk_subs(lmn, r2));
                 c[r] = pk_min(aa, bb);
@@ -333,7 +344,7 @@ template <int M, bool FUSED, bool PAD, int PF>
 __device__ __forceinline__ void sgm_line_group_buf(const SgmArgs &a, int dir, int line0, int s,
                                                    int prio)
 {
-    constexpr int NW = FUSED ? 3 + 3 * M : M; // prefetched words per step
+    constexpr int NW = FUSED ? 1 + 3 * M : M; // prefetched words per step
     const int lane = threadIdx.x & 63;
     const int line = line0 + (lane >> 4);
     const int j = lane & 15;
@@ -355,15 +366,13 @@ __device__ __forceinline__ void sgm_line_group_buf(const SgmArgs &a, int dir, in
     uint32_t oE = (uint32_t)p0 * (32 * M) + j * (2 * M);
     const uint32_t dE = (uint32_t)(step * (32 * M));
 
-    buf_t src[FUSED ? 6 : 1];
+    buf_t src[FUSED ? 4 : 1]; // packed left terms, then the three right-term planes
     uint32_t fa = 0, fb = 0, da = 0, db = 0; // fetch offsets and their steps
     if constexpr (FUSED) {
-        const uint32_t lt_plane = (uint32_t)W * H, rt_plane = (uint32_t)a.Wr * H;
+        const uint32_t rt_plane = (uint32_t)a.Wr * H;
+        src[0] = buf_rsrc(a.LT, (uint32_t)W * H * 4);
 #pragma unroll
-        for (int t = 0; t < 3; t++) {
-            src[t] = buf_rsrc(a.LT + (size_t)t * lt_plane, lt_plane * 4);
-            src[3 + t] = buf_rsrc(a.RT + (size_t)t * rt_plane, rt_plane * 4);
-        }
+        for (int t = 0; t < 3; t++) src[1 + t] = buf_rsrc(a.RT + (size_t)t * rt_plane, rt_plane * 4);
         fa = (uint32_t)p0 * 4;
         da = (uint32_t)(step * 4);
         fb = (uint32_t)(sy * a.Wr + (W - 1 - sx) + j * M) * 4;
@@ -377,10 +386,9 @@ __device__ __forceinline__ void sgm_line_group_buf(const SgmArgs &a, int dir, in
     uint32_t ring[PF][NW];
     auto fetch = [&](uint32_t(&slot)[NW]) {
         if constexpr (FUSED) {
+            slot[0] = __builtin_amdgcn_raw_buffer_load_b32(src[0], fa, 0, 0);
 #pragma unroll
-            for (int t = 0; t < 3; t++) slot[t] = __builtin_amdgcn_raw_buffer_load_b32(src[t], fa, 0, 0);
-#pragma unroll
-            for (int t = 0; t < 3; t++) buf_load<M>(src[3 + t], fb, slot + 3 + t * M);
+            for (int t = 0; t < 3; t++) buf_load<M>(src[1 + t], fb, slot + 1 + t * M);
             fb += db;
         } else {
             buf_load<M>(src[0], fa, slot);
@@ -403,10 +411,11 @@ __device__ __forceinline__ void sgm_line_group_buf(const SgmArgs &a, int dir, in
     // Costs of one step from a ring slot.
     auto costs = [&](const uint32_t(&slot)[NW], uint32_t(&c)[M]) {
         if constexpr (FUSED) {
+            uint32_t l2, lmn, lmx;
+            lt_unpack(slot[0], l2, lmn, lmx);
 #pragma unroll
             for (int r = 0; r < M; r++) {
-                const uint32_t l2 = slot[0], lmn = slot[1], lmx = slot[2];
-                const uint32_t r2 = slot[3 + r], rmn = slot[3 + M + r], rmx = slot[3 + 2 * M + r];
+                const uint32_t r2 = slot[1 + r], rmn = slot[1 + M + r], rmx = slot[1 + 2 * M + r];
                 const uint32_t aa = pk_max(pk_subs(l2, rmx), pk_subs(rmn, l2));
                 const uint32_t bb = pk_max(pk_subs(r2, lmx), pk_subs(lmn, r2));
                 c[r] = pk_min(aa, bb);
@@ -508,8 +517,8 @@ __global__ __launch_bounds__(256) void sgm_lines_kernel(const SgmArgs a, const L
 
 // ---- BT term planes for the fused path ---------------------------------------
 
-// LT[t][y][x]: splatted left terms; RT[t][y][i]: right terms of pixels
-// W-1-i (low half) and W-1-i-16m (high half), clamped at x' = 0.
+// LT[y][x]: packed left terms (lt_unpack); RT[t][y][i]: right terms of
+// pixels W-1-i (low half) and W-1-i-16m (high half), clamped at x' = 0.
 __device__ __forceinline__ void bt_planes_at(const uint8_t *__restrict__ L,
                                              const uint8_t *__restrict__ R, int W, int H,
                                              int pitch, int m, uint32_t *__restrict__ LT,
@@ -533,10 +542,8 @@ __device__ __forceinline__ void bt_planes_at(const uint8_t *__restrict__ L,
     if (i < W) {
         int v2, mn, mx;
         terms(lr, i, v2, mn, mx);
-        const size_t o = (size_t)y * W + i, pl = (size_t)W * H;
-        LT[o] = pk_splat(v2);
-        LT[pl + o] = pk_splat(mn);
-        LT[2 * pl + o] = pk_splat(mx);
+        LT[(size_t)y * W + i] =
+            (uint32_t)(v2 >> 1) | (uint32_t)(v2 - mn) << 8 | (uint32_t)(mx - v2) << 16;
     }
     if (i < Wr) {
         int a2, amn, amx, b2, bmn, bmx;
@@ -574,8 +581,9 @@ __device__ __forceinline__ void sgm_sum_pixel(const SgmArgs &a, int nplanes,
         // S = n*C + sum of E (invalid where C is)
         uint32_t c[M];
         if constexpr (FUSED) {
-            const size_t lt_plane = (size_t)W * H, rt_plane = (size_t)a.Wr * H;
-            const uint32_t lt[3] = {a.LT[p], a.LT[lt_plane + p], a.LT[2 * lt_plane + p]};
+            const size_t rt_plane = (size_t)a.Wr * H;
+            uint32_t lt[3];
+            lt_unpack(a.LT[p], lt[0], lt[1], lt[2]);
             const uint32_t *rt = a.RT + (size_t)y * a.Wr + (W - 1 - x) + j * M;
             uint32_t r2[M], rmn[M], rmx[M];
 #pragma unroll
@@ -674,8 +682,9 @@ __device__ __forceinline__ void sgm_sum_pixel_wide(const SgmArgs &a, int nplanes
     const int w0 = 8 * t;
     uint32_t c[8];
     if constexpr (FUSED) {
-        const size_t lt_plane = (size_t)W * H, rt_plane = (size_t)a.Wr * H;
-        const uint32_t l2 = a.LT[p], lmn = a.LT[lt_plane + p], lmx = a.LT[2 * lt_plane + p];
+        const size_t rt_plane = (size_t)a.Wr * H;
+        uint32_t l2, lmn, lmx;
+        lt_unpack(a.LT[p], l2, lmn, lmx);
         const uint32_t *rt = a.RT + (size_t)y * a.Wr + (W - 1 - x) + w0;
 #pragma unroll
         for (int r = 0; r < 8; r++) {
