@@ -185,7 +185,10 @@ def test_errors_are_reported(tv, dev):
                                       # long sides scanned 64 lanes per line (m = 4, 8;
                                       # horizontal and vertical; with padding cells)
                                       (200, 64, 100, 32), (50, 200, 128, 32),
-                                      (33, 300, 250, 20), (260, 90, 256, 32)])
+                                      (33, 300, 250, 20), (260, 90, 256, 32),
+                                      # odd word counts (3, 5, 7) scanned one word wider
+                                      (96, 40, 96, 32), (170, 45, 150, 32),
+                                      (120, 50, 200, 32), (230, 30, 224, 32)])
 def test_stereo_shapes_and_paths(tv, dev, fusion, W, H, D, p2):
     L, R = pair(W, H, max(D - 1, 0), 3 * W + 7 * H)
     want = oracle.wta(oracle.sgm_std(oracle.cost_bt2(L, R, D), 8, p2))

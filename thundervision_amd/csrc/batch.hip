@@ -191,7 +191,7 @@ extern "C" int tdvh_batch_create(tdvh_ctx *ctx, const tdvh_pair *pairs, int n,
         TDVH_REQUIRE(p.width > 0 && p.height > 0 && p.pitch >= p.width, "pair %d: bad image", i);
         TDVH_REQUIRE(p.depth > 0 && p.depth <= MAX_DEPTH, "pair %d: depth %d outside [1, %d]", i,
                      p.depth, MAX_DEPTH);
-        const int m = split_m(p.depth), Wr = p.width + 16 * m;
+        const int m = scan_m(p.depth), Wr = p.width + 16 * m;
         const size_t W = p.width, H = p.height;
         off_terms[i] = total;
         total = align(total + (W * H + 3 * (size_t)Wr * H) * 4);
@@ -215,7 +215,7 @@ extern "C" int tdvh_batch_create(tdvh_ctx *ctx, const tdvh_pair *pairs, int n,
         std::vector<int> cls;
         int64_t cls_cells = 0;
         for (int i = 0; i < n; i++)
-            if (split_m(pairs[i].depth) == mc) {
+            if (scan_m(pairs[i].depth) == mc) {
                 cls.push_back(i);
                 cls_cells += (int64_t)pairs[i].width * pairs[i].height * pairs[i].depth;
             }
@@ -307,7 +307,7 @@ extern "C" int tdvh_batch_create(tdvh_ctx *ctx, const tdvh_pair *pairs, int n,
     for (int i = 0; i < n; i++) {
         const tdvh_pair &p = pairs[i];
         BPair &q = bp[i];
-        const int m = split_m(p.depth);
+        const int m = scan_m(p.depth);
         q.a = SgmArgs{};
         q.a.W = p.width;
         q.a.H = p.height;
@@ -315,6 +315,7 @@ extern "C" int tdvh_batch_create(tdvh_ctx *ctx, const tdvh_pair *pairs, int n,
         q.a.P1 = params->p1;
         q.a.P2 = params->p2;
         q.a.C = nullptr;
+        q.a.m = m;
         q.a.Wr = p.width + 16 * m;
         q.a.LT = (const uint32_t *)(base + off_terms[i]);
         q.a.RT = q.a.LT + (size_t)p.width * p.height;

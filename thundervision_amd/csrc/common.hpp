@@ -58,6 +58,15 @@ constexpr int SPLIT_LANES = 16;    // lanes per disparity vector (one DPP row)
 // words per pixel of the SPLIT16 layout: 16 * m, m = ceil(D / 32)
 inline int split_m(int depth) { return (depth + 31) / 32; }
 inline int split_words(int depth) { return 16 * split_m(depth); }
+// Words per lane of the fused line scans and plane sums, where no SPLIT16
+// volume fixes the shape: odd counts above 1 run one word wider (padding
+// cells d >= D), which the scans and sums handle faster than the odd shape
+// (measured on 640x480 and 1280x720 pairs).
+inline int scan_m(int depth)
+{
+    const int m = split_m(depth);
+    return (m == 3 || m == 5 || m == 7) ? m + 1 : m;
+}
 
 // Direction index -> (dx, dy): 0 (1,0) 1 (-1,0) 2 (0,1) 3 (0,-1) 4 (1,1)
 // 5 (-1,-1) 6 (-1,1) 7 (1,-1), the order of the oracle and of tdvh_sgm_params.

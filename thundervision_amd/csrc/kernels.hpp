@@ -14,7 +14,7 @@ int launch_convert(uint16_t *strided, int64_t sx, int64_t sy, uint32_t *split, i
 
 // BT term planes of the fused SGM path (splatted left terms, paired right
 // terms; see sgm.hip).
-int launch_bt_planes(const uint8_t *L, const uint8_t *R, int W, int H, int pitch, int D,
+int launch_bt_planes(const uint8_t *L, const uint8_t *R, int W, int H, int pitch, int m,
                      uint32_t *LT, uint32_t *RT, hipStream_t s);
 
 // Reference-geometry SGM into an int32 strided [y][x][d] volume.

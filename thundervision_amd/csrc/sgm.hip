@@ -222,7 +222,7 @@ int launch_sgm_lines(tdvh_ctx *ctx, const SgmArgs &a, int dir_mask, bool eout, h
         TDVH_TRY(line_order(ctx, a.W, a.H, dir_mask, &order, &groups));
         prof_begin(s);
         const bool fused = a.C == nullptr;
-        switch (split_m(a.D)) {
+        switch (a.m) {
 #define CASE(m)                                                                                    \
     case m:                                                                                        \
         if (fused) launch_lines_buf<m, true>(a, order, groups, ls.prio, s);                        \
@@ -235,7 +235,7 @@ int launch_sgm_lines(tdvh_ctx *ctx, const SgmArgs &a, int dir_mask, bool eout, h
         return check_launch("sgm_lines_kernel", s);
     }
     prof_begin(s);
-    switch (split_m(a.D)) {
+    switch (a.m) {
 #define CASE(m)                                                                                    \
     case m: launch_lines_m<m>(a, ls, eout, s); break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
@@ -255,10 +255,10 @@ __global__ __launch_bounds__(256) void bt_planes_kernel(const uint8_t *__restric
                  blockIdx.y);
 }
 
-int launch_bt_planes(const uint8_t *L, const uint8_t *R, int W, int H, int pitch, int D,
+int launch_bt_planes(const uint8_t *L, const uint8_t *R, int W, int H, int pitch, int m,
                      uint32_t *LT, uint32_t *RT, hipStream_t s)
 {
-    const int m = split_m(D), Wr = W + 16 * m;
+    const int Wr = W + 16 * m;
     dim3 grid((Wr + 255) / 256, H);
     prof_begin(s);
     bt_planes_kernel<<<grid, 256, 0, s>>>(L, R, W, H, pitch, m, LT, RT, Wr);
@@ -306,7 +306,7 @@ int launch_sgm_sum_wta(const SgmArgs &a, int nplanes, bool eout, uint32_t *S_spl
     const long pixels = (long)a.W * a.H;
     const int blocks = (int)((pixels + 15) / 16);
     prof_begin(s);
-    switch (split_m(a.D)) {
+    switch (a.m) {
 #define CASE(m)                                                                                    \
     case m: launch_sum_m<m>(a, nplanes, eout, S_split, disp, disp_pitch, blocks, s); break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
@@ -338,9 +338,11 @@ static int sgm_std(tdvh_ctx *ctx, const uint32_t *Csplit, const uint8_t *left,
                    hipStream_t s)
 {
     const int nplanes = popcount8(params->dir_mask);
-    const int m = split_m(D);
     // path-cost planes: L - C in bytes when P2 fits a byte, else L words
     const bool eout = params->p2 <= 255;
+    // a SPLIT16 volume (C in, S out, or the staged fallback) fixes the
+    // register shape; the fused byte path picks its own
+    const int m = (!Csplit && eout && !S_out) ? scan_m(D) : split_m(D);
     SgmArgs a{};
     a.W = W;
     a.H = H;
@@ -348,6 +350,7 @@ static int sgm_std(tdvh_ctx *ctx, const uint32_t *Csplit, const uint8_t *left,
     a.P1 = params->p1;
     a.P2 = params->p2;
     a.C = Csplit;
+    a.m = m;
     a.Wr = W + 16 * m;
     a.plane_bytes = (size_t)W * H * 32 * m * (eout ? 1 : 2);
     void *pb = nullptr;
@@ -362,7 +365,7 @@ static int sgm_std(tdvh_ctx *ctx, const uint32_t *Csplit, const uint8_t *left,
         TDVH_TRY(ctx->scratch("bt_planes", words * 4, &tb));
         a.LT = (const uint32_t *)tb;
         a.RT = (const uint32_t *)tb + (size_t)W * H;
-        TDVH_TRY(launch_bt_planes(left, right, W, H, pitch, D, (uint32_t *)a.LT,
+        TDVH_TRY(launch_bt_planes(left, right, W, H, pitch, m, (uint32_t *)a.LT,
                                   (uint32_t *)a.RT, s));
         if (!eout) {
             // L words need C in the sum: fall back to a staged volume

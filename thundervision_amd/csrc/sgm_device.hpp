@@ -45,6 +45,7 @@ __host__ __device__ inline void line_geom(int dir, int line, int W, int H, int &
 // Device-side inputs of the SGM kernels.
 struct SgmArgs {
     int W, H, D, P1, P2;
+    int m;              // words per lane and plane (>= ceil(D/32); cells d >= D are padding)
     const uint32_t *C;  // SPLIT16 cost volume (staged), or null
     const uint32_t *LT; // W*H packed left BT terms (lt_unpack) (fused)
     const uint32_t *RT; // 3 planes H*Wr: paired right BT terms r2, rmin, rmax (fused)
@@ -499,9 +500,8 @@ sgm_lines_buf_kernel(const SgmArgs a, const uint32_t *__restrict__ order, int pr
 __host__ __device__ inline bool buf_lines_fit(const SgmArgs &a)
 {
     const size_t lim = (size_t)1 << 31;
-    const size_t m = (size_t)((a.D + 31) / 32);
     if (a.plane_bytes >= lim) return false;
-    if (a.C) return (size_t)a.W * a.H * 64 * m < lim;
+    if (a.C) return (size_t)a.W * a.H * 64 * (size_t)a.m < lim;
     return (size_t)a.W * a.H * 4 < lim && (size_t)a.Wr * a.H * 4 < lim;
 }
 
