@@ -151,12 +151,15 @@ struct tdvh_batch {
     std::vector<hipEvent_t> seg_done; // per segment: its line scans finished
 };
 
-// Segments per depth class (TDVH_BATCH_SEGMENTS, default 4).
+// Segments per depth class (TDVH_BATCH_SEGMENTS, default 1). Measured on the
+// 64-pair mix: overlapping a segment's plane sums with the next segment's
+// line scans (4 segments) ran ~10 % slower than one segment per class, as
+// both phases then compete for the same memory pipeline and HBM.
 static int batch_segments()
 {
     static const int n = [] {
         const char *e = getenv("TDVH_BATCH_SEGMENTS");
-        const int v = e ? atoi(e) : 4;
+        const int v = e ? atoi(e) : 1;
         return v < 1 ? 1 : (v > 16 ? 16 : v);
     }();
     return n;
