@@ -72,8 +72,8 @@ batch_lines_buf_kernel(const BPair *__restrict__ pairs, const BItem *__restrict_
     sgm_line_group_buf<M, true, PAD, PF>(pairs[it.pair].a, it.a, it.b, it.a, prio);
 }
 
-// Plane sum + WTA with 16-byte lanes (2M lanes per pixel, 64/2M pixels per
-// wave, one pass), for power-of-two word counts. items: one per pair of the
+// Plane sum + WTA with 16-byte lanes (sum_group lanes per pixel, one
+// pass), for word counts 1, 2, 4, 6 and 8. items: one per pair of the
 // class, {pair, 3, first wave, waves}, in increasing first wave; a wave finds
 // its pair by binary search (wave-uniform, scalar loads).
 template <int M>
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) void batch_sum_wide_kernel(const BPair *__rest
                                                              const BItem *__restrict__ items,
                                                              int n, int waves)
 {
-    constexpr int G = 2 * M;
+    constexpr int G = sum_group<M>();
     const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (w >= waves) return;
     int lo = 0, hi = n - 1;
@@ -269,13 +269,13 @@ extern "C" int tdvh_batch_create(tdvh_ctx *ctx, const tdvh_pair *pairs, int n,
             for (auto &l : lines) items.push_back(l.second);
             sg.line_count = (int)items.size() - sg.line_begin;
             sg.sum_begin = (int)items.size();
-            if (mc == 1 || mc == 2 || mc == 4 || mc == 8) {
+            if (mc == 1 || mc == 2 || mc == 4 || mc == 6 || mc == 8) {
                 // one item per pair: {pair, 3, first wave, waves} (wide sum)
                 int wave = 0;
                 for (size_t c = c0; c < c1; c++) {
                     const int i = cls[c];
                     const long npix = (long)pairs[i].width * pairs[i].height;
-                    const int nw = (int)((npix * 2 * mc + 63) / 64);
+                    const int nw = (int)((npix * (mc == 6 ? 16 : 2 * mc) + 63) / 64);
                     items.push_back({i, 3, wave, nw});
                     wave += nw;
                 }
@@ -442,7 +442,7 @@ static void launch_sum_seg(tdvh_batch *b, const tdvh_batch::Seg &sg, hipStream_t
     if (!ns) return;
     const BItem *si = b->d_items + sg.sum_begin;
     prof_begin(s);
-    if constexpr (M == 1 || M == 2 || M == 4 || M == 8)
+    if constexpr (sum_wide_ok<M>())
         batch_sum_wide_kernel<M><<<(sg.sum_waves + 3) / 4, 256, 0, s>>>(b->d_pairs, si, ns,
                                                                         sg.sum_waves);
     else

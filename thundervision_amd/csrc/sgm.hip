@@ -281,9 +281,9 @@ static void launch_sum_m(const SgmArgs &a, int nplanes, bool eout, uint32_t *S, 
                          int pitch, int blocks, hipStream_t s)
 {
     const bool fused = a.C == nullptr;
-    if constexpr (M == 1 || M == 2 || M == 4 || M == 8) {
+    if constexpr (sum_wide_ok<M>()) {
         if (eout && sum_wide_enabled()) {
-            const long threads = (long)a.W * a.H * 2 * M;
+            const long threads = (long)a.W * a.H * sum_group<M>();
             const int wb = (int)((threads + 255) / 256);
             if (fused)
                 sgm_sum_wide_kernel<M, true><<<wb, 256, 0, s>>>(a, nplanes, S, disp, pitch);

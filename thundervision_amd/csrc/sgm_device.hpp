@@ -663,8 +663,13 @@ __device__ __forceinline__ uint32_t group_min_u32(uint32_t v)
     return v;
 }
 
-// Plane sum + lowest-index WTA with 16-byte lanes: lane t of a pixel's
-// group of G = 2M lanes owns SPLIT words w = 8t .. 8t+7 (cells d = w in the
+// Word counts with a wide plane sum, and the lanes per pixel of that sum:
+// 2M (a DPP-aligned power of two), or 16 for M = 6 (12 working lanes).
+template <int M> constexpr bool sum_wide_ok() { return M == 1 || M == 2 || M == 4 || M == 6 || M == 8; }
+template <int M> constexpr int sum_group() { return M == 6 ? 16 : 2 * M; }
+
+// Plane sum + lowest-index WTA with 16-byte lanes: lane t < 2M of a pixel's
+// group of G lanes owns SPLIT words w = 8t .. 8t+7 (cells d = w in the
 // low halves, d = 16M + w in the high halves), i.e. bytes 16t .. 16t+15 of
 // the pixel in every plane, so each plane is read in whole 1 KiB wave rows.
 // S = n*C + sum of E (invalid where C is), then the key (S << 9 | d)
@@ -675,79 +680,81 @@ __device__ __forceinline__ void sgm_sum_pixel_wide(const SgmArgs &a, int nplanes
                                                    int16_t *__restrict__ disp, int pitch, long p,
                                                    int t)
 {
-    static_assert(M == 1 || M == 2 || M == 4 || M == 8, "power-of-two word counts");
-    constexpr int G = 2 * M;
+    static_assert(sum_wide_ok<M>(), "word counts 1, 2, 4, 6, 8");
+    constexpr int G = sum_group<M>();
     const int W = a.W, H = a.H;
     const int y = (int)(p / W), x = (int)(p - (long)y * W);
     const int w0 = 8 * t;
-    uint32_t c[8];
-    if constexpr (FUSED) {
-        const size_t rt_plane = (size_t)a.Wr * H;
-        uint32_t l2, lmn, lmx;
-        lt_unpack(a.LT[p], l2, lmn, lmx);
-        const uint32_t *rt = a.RT + (size_t)y * a.Wr + (W - 1 - x) + w0;
+    uint32_t key = 0xFFFFFFFFu;
+    if (G == 2 * M || t < 2 * M) {
+        uint32_t c[8];
+        if constexpr (FUSED) {
+            const size_t rt_plane = (size_t)a.Wr * H;
+            uint32_t l2, lmn, lmx;
+            lt_unpack(a.LT[p], l2, lmn, lmx);
+            const uint32_t *rt = a.RT + (size_t)y * a.Wr + (W - 1 - x) + w0;
 #pragma unroll
-        for (int r = 0; r < 8; r++) {
-            const uint32_t r2 = rt[r], rmn = rt[rt_plane + r], rmx = rt[2 * rt_plane + r];
-            const uint32_t aa = pk_max(pk_subs(l2, rmx), pk_subs(rmn, l2));
-            const uint32_t bb = pk_max(pk_subs(r2, lmx), pk_subs(lmn, r2));
-            c[r] = pk_min(aa, bb);
+            for (int r = 0; r < 8; r++) {
+                const uint32_t r2 = rt[r], rmn = rt[rt_plane + r], rmx = rt[2 * rt_plane + r];
+                const uint32_t aa = pk_max(pk_subs(l2, rmx), pk_subs(rmn, l2));
+                const uint32_t bb = pk_max(pk_subs(r2, lmx), pk_subs(lmn, r2));
+                c[r] = pk_min(aa, bb);
+            }
+        } else {
+            const uint32_t *src = a.C + (size_t)p * (16 * M) + w0;
+#pragma unroll
+            for (int r = 0; r < 8; r++) c[r] = src[r];
         }
-    } else {
-        const uint32_t *src = a.C + (size_t)p * (16 * M) + w0;
+        if (a.D < 32 * M) {
 #pragma unroll
-        for (int r = 0; r < 8; r++) c[r] = src[r];
-    }
-    if (a.D < 32 * M) {
-#pragma unroll
-        for (int r = 0; r < 8; r++) {
-            const int w = w0 + r;
-            c[r] |= (w >= a.D ? 0x0000FFFFu : 0u) | (16 * M + w >= a.D ? 0xFFFF0000u : 0u);
-        }
-    }
-    // E bytes are summed four at a time in 32-bit adds while no byte can
-    // carry (g planes with g * P2 <= 255), then widened into uint16 pairs.
-    uint32_t acc[8];
-#pragma unroll
-    for (int r = 0; r < 8; r++) acc[r] = 0;
-    const uint8_t *e = a.planes + (size_t)p * (32 * M) + 16 * t;
-    tdvh_u32x4 ev[8];
-#pragma unroll
-    for (int q = 0; q < 8; q++)
-        if (q < nplanes)
-            ev[q] = __builtin_nontemporal_load((const tdvh_u32x4 *)(e + q * a.plane_bytes));
-    const int g = a.P2 > 0 ? min(255 / a.P2, 8) : 8;
-    uint32_t bs[4] = {0, 0, 0, 0};
-    int cnt = 0;
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-        if (q < nplanes) {
-            bs[0] += ev[q].x, bs[1] += ev[q].y, bs[2] += ev[q].z, bs[3] += ev[q].w;
-            if (++cnt == g || q == nplanes - 1) {
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    acc[2 * i] = pk_add(acc[2 * i], unpack_e(bs[i], 0));
-                    acc[2 * i + 1] = pk_add(acc[2 * i + 1], unpack_e(bs[i], 1));
-                    bs[i] = 0;
-                }
-                cnt = 0;
+            for (int r = 0; r < 8; r++) {
+                const int w = w0 + r;
+                c[r] |= (w >= a.D ? 0x0000FFFFu : 0u) | (16 * M + w >= a.D ? 0xFFFF0000u : 0u);
             }
         }
-    }
-    const uint32_t n = pk_splat((uint32_t)nplanes);
-    uint32_t key = 0xFFFFFFFFu;
+        // E bytes are summed four at a time in 32-bit adds while no byte can
+        // carry (g planes with g * P2 <= 255), then widened into uint16 pairs.
+        uint32_t acc[8];
 #pragma unroll
-    for (int r = 0; r < 8; r++) {
-        const uint32_t inv = pk_sub(0u, u32(v2(c[r]) >> (tdvh_u16x2)(15))); // 0xFFFF where C invalid
-        acc[r] = pk_add(u32(v2(c[r]) * v2(n)), acc[r]) | inv;
-        const uint32_t w = (uint32_t)(w0 + r);
-        key = min(key, ((acc[r] & 0xFFFFu) << 9) | w);
-        key = min(key, ((acc[r] >> 16) << 9) | (16 * M + w));
-    }
-    if (S) {
-        uint4 *dst = (uint4 *)(S + (size_t)p * (16 * M) + w0);
-        dst[0] = make_uint4(acc[0], acc[1], acc[2], acc[3]);
-        dst[1] = make_uint4(acc[4], acc[5], acc[6], acc[7]);
+        for (int r = 0; r < 8; r++) acc[r] = 0;
+        const uint8_t *e = a.planes + (size_t)p * (32 * M) + 16 * t;
+        tdvh_u32x4 ev[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            if (q < nplanes)
+                ev[q] = __builtin_nontemporal_load((const tdvh_u32x4 *)(e + q * a.plane_bytes));
+        const int g = a.P2 > 0 ? min(255 / a.P2, 8) : 8;
+        uint32_t bs[4] = {0, 0, 0, 0};
+        int cnt = 0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            if (q < nplanes) {
+                bs[0] += ev[q].x, bs[1] += ev[q].y, bs[2] += ev[q].z, bs[3] += ev[q].w;
+                if (++cnt == g || q == nplanes - 1) {
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        acc[2 * i] = pk_add(acc[2 * i], unpack_e(bs[i], 0));
+                        acc[2 * i + 1] = pk_add(acc[2 * i + 1], unpack_e(bs[i], 1));
+                        bs[i] = 0;
+                    }
+                    cnt = 0;
+                }
+            }
+        }
+        const uint32_t n = pk_splat((uint32_t)nplanes);
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            const uint32_t inv = pk_sub(0u, u32(v2(c[r]) >> (tdvh_u16x2)(15))); // 0xFFFF where C invalid
+            acc[r] = pk_add(u32(v2(c[r]) * v2(n)), acc[r]) | inv;
+            const uint32_t w = (uint32_t)(w0 + r);
+            key = min(key, ((acc[r] & 0xFFFFu) << 9) | w);
+            key = min(key, ((acc[r] >> 16) << 9) | (16 * M + w));
+        }
+        if (S) {
+            uint4 *dst = (uint4 *)(S + (size_t)p * (16 * M) + w0);
+            dst[0] = make_uint4(acc[0], acc[1], acc[2], acc[3]);
+            dst[1] = make_uint4(acc[4], acc[5], acc[6], acc[7]);
+        }
     }
     key = group_min_u32<G>(key);
     if (t == 0) disp[(size_t)y * pitch + x] = (int16_t)(key & 0x1FFu);
@@ -758,7 +765,7 @@ __global__ __launch_bounds__(256) void sgm_sum_wide_kernel(const SgmArgs a, int 
                                                            uint32_t *__restrict__ S,
                                                            int16_t *__restrict__ disp, int pitch)
 {
-    constexpr int G = 2 * M;
+    constexpr int G = sum_group<M>();
     const long p = ((long)blockIdx.x * 256 + threadIdx.x) / G;
     if (p >= (long)a.W * a.H) return; // whole groups leave together (256 % G == 0)
     sgm_sum_pixel_wide<M, FUSED>(a, nplanes, S, disp, pitch, p, threadIdx.x % G);
