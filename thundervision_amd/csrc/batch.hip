@@ -409,8 +409,8 @@ static void launch_lines_seg(tdvh_batch *b, const tdvh_batch::Seg &sg, hipStream
     const int blocks = (nl + wpb - 1) / wpb;
     prof_begin(s);
     if (b->class_buf[M]) {
-        // prefetch 8 only where its ring fits two waves per SIMD
-        constexpr int PF8 = M <= 4 ? 8 : 4;
+        // "prefetch 8": the deepest ring that keeps two waves per SIMD
+        constexpr int PF8 = buf_lines_pf<M, true>();
         if (b->class_pad[M]) {
             if (b->prefetch == 8)
                 batch_lines_buf_kernel<M, true, PF8><<<blocks, threads, 0, s>>>(b->d_pairs, li, nl,

@@ -478,11 +478,16 @@ __device__ __forceinline__ void sgm_line_group_buf(const SgmArgs &a, int dir, in
         if (k0 + u < kmax) body(u, k0 + u);
 }
 
-// Prefetch depth of the buffer line kernel: 8 steps for up to 4 words per
-// lane and plane (<= 248 VGPRs), 4 above; two waves per SIMD. Fewer waves
-// per SIMD shorten the longest lines (the launch's critical path), which
-// otherwise share their SIMD's issue slots with two or three other waves.
-template <int M> constexpr int buf_lines_pf() { return M <= 4 ? 8 : 4; }
+// Prefetch depth of the buffer line kernel, the deepest ring that keeps two
+// waves per SIMD without spills: fused, 8 steps up to 4 words per lane and
+// plane, 6 at 6 words, 4 at 8; staged (M words per step), 8. Measured on
+// 1280x720: a deeper ring with two waves per SIMD beats a shallower one with
+// three or four waves (the longest lines, the launch's critical path, then
+// share their SIMD's issue slots with fewer waves).
+template <int M, bool FUSED> constexpr int buf_lines_pf()
+{
+    return !FUSED || M <= 4 ? 8 : M == 6 ? 6 : 4;
+}
 
 // One wave per workgroup, so the dispatcher starts the next line group as
 // soon as any wave ends; `order` lists the groups longest first (entry:
@@ -492,7 +497,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void
 sgm_lines_buf_kernel(const SgmArgs a, const uint32_t *__restrict__ order, int prio)
 {
     const uint32_t e = order[blockIdx.x];
-    sgm_line_group_buf<M, FUSED, PAD, buf_lines_pf<M>()>(a, (int)((e >> 24) & 0xF),
+    sgm_line_group_buf<M, FUSED, PAD, buf_lines_pf<M, FUSED>()>(a, (int)((e >> 24) & 0xF),
                                                          (int)(e & 0xFFFFFF), (int)(e >> 28), prio);
 }
 
