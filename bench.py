@@ -167,6 +167,47 @@ def bench_served(torch, duration=1.0, rate=600.0, frames_per_stream=4, slots=4):
     return out
 
 
+def bench_window(tv, synth, torch, ctx, reps=5):
+    """The window-cost paths at 1920x1080, D = 256 (seed 3): SSD 7x7 (uint32)
+    -> DP (P_occ = 20), 9x9 zero-mean NCC (fp32) -> WTA, and the fused SSD +
+    WTA without a volume. B_alg: 9 B/cell for SSD -> DP (cost write + read
+    at uint32, 1 B path code), 8 B/cell for NCC -> WTA."""
+    spec = synth.WINDOW_1080P
+    W, H, D = spec["width"], spec["height"], spec["depth"]
+    L, R, _ = synth.workload_pair(spec)
+    dev = torch.device("cuda:0")
+    Ld, Rd = torch.from_numpy(L).to(dev), torch.from_numpy(R).to(dev)
+    cells = W * H * D
+    paths = {
+        "ssd_dp": (lambda: tv.dp(tv.cost_ssd(Ld, Rd, D, radius=3, ctx=ctx), p_occ=20, ctx=ctx),
+                   9),
+        "ncc_wta": (lambda: tv.wta(tv.cost_ncc(Ld, Rd, D, radius=4, zero_mean=True, ctx=ctx),
+                                   ctx=ctx), 8),
+        "fused_ssd_wta": (lambda: tv.fast_wta_ssd(Ld, Rd, D, radius=3, ctx=ctx), None),
+    }
+    out = {"workload": "1920x1080 uint8 pair, D=256, seed 3"}
+    for name, (fn, bpc) in paths.items():
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        ctx.profile(True)
+        ctx.profile_reset()
+        fn()
+        torch.cuda.synchronize()
+        table = ctx.profile_table()
+        ctx.profile(False)
+        row = {"ms": round(ms, 3), "MDE/s": round(cells / (ms * 1e-3) / 1e6, 1),
+               "kernels_ms": {k: round(v[0] / v[1], 4) for k, v in table.items()}}
+        if bpc:
+            row["frac_of_hbm_roofline"] = round(cells * bpc / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        out[name] = row
+    return out
+
+
 def bench_boundary(tv, torch, ctx, L, R, D, reps=20):
     """Steps around the path at the headline shape (device-resident unless
     stated): Gaussian prefilter, erode/dilate postfilter, colour to gray,
@@ -350,6 +391,7 @@ def main():
         served = bench_served(torch)
 
     boundary = bench_boundary(tv, torch, ctx, L, R, D)
+    window = bench_window(tv, synth, torch, ctx)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -399,6 +441,7 @@ def main():
         "batched_720p": batched_hd,
         "served": served,
         "boundary": boundary,
+        "window_1080p": window,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

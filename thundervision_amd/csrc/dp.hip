@@ -10,10 +10,14 @@
 // Accumulation is uint64, so no row can overflow (1920 * 49 * 255^2 > 2^32).
 //
 // Mapping: one wave per row, lane l holds d = l*K + k (K <= 4) in registers;
-// the d-1 / d+1 neighbours across lanes move with ds_bpermute shuffles. Each
-// step stores the lane's K two-bit decisions as one byte (64 bytes per
-// column). The backtrack stages 64 columns of decisions in LDS at a time and
-// walks them with a wave-uniform disparity.
+// the d-1 / d+1 neighbours across lanes move with whole-wave DPP shifts (the
+// vacated end lane reads the invalid value, which is exactly the missing
+// neighbour of d = 0 and of d = 64K-1; cells d >= D carry invalid costs and
+// stay invalid, so no disparity bound needs a test). Costs stream through a
+// PF-step prefetch ring (one wave per SIMD leaves nothing else to hide the
+// load latency). Each step stores the lane's K two-bit decisions as one byte
+// (64 bytes per column). The backtrack stages 64 columns of decisions in LDS
+// at a time and walks them with a wave-uniform disparity.
 #include "kernels.hpp"
 
 namespace tdvh {
@@ -27,12 +31,33 @@ __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src)
     return ((uint64_t)hi << 32) | lo;
 }
 
-template <int K, int PF>
+// Whole-wave DPP shifts of a 64-bit value: wave_shr:1 (lane i reads lane
+// i-1) and wave_shl:1 (lane i reads lane i+1); the lane without a source
+// reads DP_INF.
+__device__ __forceinline__ uint64_t wave_from_prev(uint64_t v)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)(uint32_t)v, 0x138, 0xF, 0xF, false);
+    const uint32_t hi =
+        (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)(uint32_t)(v >> 32), 0x138, 0xF, 0xF, false);
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t wave_from_next(uint64_t v)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)(uint32_t)v, 0x130, 0xF, 0xF, false);
+    const uint32_t hi =
+        (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)(uint32_t)(v >> 32), 0x130, 0xF, 0xF, false);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// VEC: K == 4, D == 256 and 16-byte aligned lanes (one 16-byte load per lane
+// and step); otherwise K word loads with a disparity bound.
+template <int K, int PF, bool VEC>
 __global__ __launch_bounds__(64) void dp_kernel(const uint32_t *__restrict__ C, int W, int H,
                                                 int D, int64_t sx, int64_t sy, uint32_t P,
                                                 int flags, uint8_t *__restrict__ path,
                                                 int16_t *__restrict__ disp, int pitch)
 {
+    static_assert(!VEC || K == 4, "vector loads cover four cells");
     __shared__ uint8_t sp[64][64];
     const int l = threadIdx.x;
     const int y = blockIdx.x;
@@ -41,11 +66,13 @@ __global__ __launch_bounds__(64) void dp_kernel(const uint32_t *__restrict__ C, 
 
     auto load = [&](uint32_t(&c)[K], int x) {
         const int xx = x < W ? x : W - 1;
-        const uint32_t *src = crow + xx * sx;
+        const uint32_t *src = crow + xx * sx + l * K;
+        if constexpr (VEC) {
+            const uint4 v = *(const uint4 *)src;
+            c[0] = v.x, c[1] = v.y, c[2] = v.z, c[3] = v.w;
+        } else {
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            const int d = l * K + k;
-            c[k] = d < D ? src[d] : INV32;
+            for (int k = 0; k < K; k++) c[k] = l * K + k < D ? src[k] : INV32;
         }
     };
 
@@ -60,50 +87,52 @@ __global__ __launch_bounds__(64) void dp_kernel(const uint32_t *__restrict__ C, 
 #pragma unroll
     for (int u = 0; u < PF; u++) load(ring[u], 1 + u);
 
-    for (int x0 = 1; x0 < W; x0 += PF) {
+    // one step at column x from ring slot u (which it then refills)
+    auto step = [&](int u, int x) {
+        uint32_t c[K];
 #pragma unroll
-        for (int u = 0; u < PF; u++) {
-            const int x = x0 + u;
-            if (x < W) {
-                uint32_t c[K];
+        for (int k = 0; k < K; k++) c[k] = ring[u][k];
+        load(ring[u], x + PF);
+        const uint64_t left = wave_from_prev(A[K - 1]);
+        const uint64_t right = wave_from_next(A[0]);
+        uint32_t codes = 0;
+        uint64_t An[K];
 #pragma unroll
-                for (int k = 0; k < K; k++) c[k] = ring[u][k];
-                load(ring[u], x + PF);
-                uint64_t left = shfl_u64(A[K - 1], l - 1);
-                uint64_t right = shfl_u64(A[0], l + 1);
-                if (l == 0) left = DP_INF;
-                if (l == 63) right = DP_INF;
-                uint32_t codes = 0;
-                uint64_t An[K];
-#pragma unroll
-                for (int k = 0; k < K; k++) {
-                    const int d = l * K + k;
-                    const uint64_t lm = k == 0 ? left : A[k - 1];
-                    const uint64_t lp = k == K - 1 ? right : A[k + 1];
-                    const uint64_t c1 = (d > 0 && lm != DP_INF) ? lm + P : DP_INF;
-                    const uint64_t c2 = A[k];
-                    const uint64_t c3 = (d < D - 1 && lp != DP_INF) ? lp + P : DP_INF;
-                    uint64_t mv;
-                    uint32_t code;
-                    if (c1 < c2 && c1 < c3) {
-                        mv = c1;
-                        code = 0;
-                    } else if (c2 < c3) {
-                        mv = c2;
-                        code = 1;
-                    } else {
-                        mv = c3;
-                        code = 2;
-                    }
-                    An[k] = (c[k] == INV32 || mv == DP_INF) ? DP_INF : mv + c[k];
-                    codes |= code << (2 * k);
-                }
-#pragma unroll
-                for (int k = 0; k < K; k++) A[k] = An[k];
-                prow[(size_t)x * 64 + l] = (uint8_t)codes;
+        for (int k = 0; k < K; k++) {
+            const uint64_t lm = k == 0 ? left : A[k - 1];
+            const uint64_t lp = k == K - 1 ? right : A[k + 1];
+            const uint64_t c1 = lm != DP_INF ? lm + P : DP_INF;
+            const uint64_t c2 = A[k];
+            const uint64_t c3 = lp != DP_INF ? lp + P : DP_INF;
+            uint64_t mv;
+            uint32_t code;
+            if (c1 < c2 && c1 < c3) {
+                mv = c1;
+                code = 0;
+            } else if (c2 < c3) {
+                mv = c2;
+                code = 1;
+            } else {
+                mv = c3;
+                code = 2;
             }
+            An[k] = (c[k] == INV32 || mv == DP_INF) ? DP_INF : mv + c[k];
+            codes |= code << (2 * k);
         }
+#pragma unroll
+        for (int k = 0; k < K; k++) A[k] = An[k];
+        prow[(size_t)x * 64 + l] = (uint8_t)codes;
+    };
+    // whole blocks of PF steps without conditions (so the waits stay per
+    // slot), then the tail
+    int x0 = 1;
+    for (; x0 + PF <= W; x0 += PF) {
+#pragma unroll
+        for (int u = 0; u < PF; u++) step(u, x0 + u);
     }
+#pragma unroll
+    for (int u = 0; u < PF; u++)
+        if (x0 + u < W) step(u, x0 + u);
 
     // start: lowest-index argmin of the last column (or d = 0)
     int start = 0;
@@ -175,15 +204,27 @@ extern "C" int tdvh_dp(tdvh_ctx *ctx, const tdvh_dsi *C, uint32_t p_occ, int fla
     TDVH_TRY(ctx->scratch("dp_path", (size_t)W * H * 64, &pb));
     const int K = (D + 63) / 64;
     prof_begin(s);
+    const uint32_t *cp = (const uint32_t *)C->ptr;
+    const bool vec = D == 256 && C->stride_x % 4 == 0 && C->stride_y % 4 == 0 &&
+                     ((uintptr_t)cp & 15) == 0;
+    constexpr int PF = 16;
     switch (K) {
 #define CASE(k)                                                                                    \
     case k:                                                                                        \
-        dp_kernel<k, 4><<<H, 64, 0, s>>>((const uint32_t *)C->ptr, W, H, D, C->stride_x,           \
-                                         C->stride_y, p_occ, flags, (uint8_t *)pb, disp,           \
-                                         disp_pitch);                                              \
+        dp_kernel<k, PF, false><<<H, 64, 0, s>>>(cp, W, H, D, C->stride_x, C->stride_y, p_occ,     \
+                                                 flags, (uint8_t *)pb, disp, disp_pitch);          \
         break;
-        CASE(1) CASE(2) CASE(3) CASE(4)
+        CASE(1) CASE(2) CASE(3)
 #undef CASE
+    case 4:
+        if (vec)
+            dp_kernel<4, PF, true><<<H, 64, 0, s>>>(cp, W, H, D, C->stride_x, C->stride_y, p_occ,
+                                                    flags, (uint8_t *)pb, disp, disp_pitch);
+        else
+            dp_kernel<4, PF, false><<<H, 64, 0, s>>>(cp, W, H, D, C->stride_x, C->stride_y,
+                                                     p_occ, flags, (uint8_t *)pb, disp,
+                                                     disp_pitch);
+        break;
     }
     return check_launch("dp_kernel", s);
 }
