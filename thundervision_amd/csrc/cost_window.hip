@@ -1,26 +1,35 @@
 // Window matching costs: SSD (ssd.cu:15-50), NCC (crosscorrelation.cu:11-49)
 // and the fused SSD + lowest-index WTA of fastwta.cu:32-53.
 //
-// Mapping: a block covers 4 consecutive x (one wave each) of a strip of
-// output rows; lane i of a wave holds disparity c*64 + i of chunk c, so the
-// 64 lanes write 64 consecutive d of a pixel (coalesced, d-contiguous
-// layout). Image rows are staged in LDS one row at a time; each thread keeps
-// the per-row window terms of its (x, d) cells in a register ring and slides
-// the (2r+1)-row window down the strip, so a cell costs one row term
-// (2r+1 products) instead of (2r+1)^2.
+// Mapping: one wave per tile of WIN_TX consecutive x and WIN_TY output rows;
+// lane i holds disparity c*64 + i of every chunk c, so the 64 lanes write 64
+// consecutive d of a pixel (coalesced, d-contiguous layout). The box sum is
+// separable and slides both ways: per chunk each lane keeps, for the tile's
+// WIN_TX + 2r columns, the vertical window sum of the pixel term
+//   SSD: (L(x', y') - R(x' - d, y'))^2     NCC: L(x', y') * R(x' - d, y')
+// over rows y-r .. y+r; one output row adds the entering row's terms and
+// subtracts the leaving row's, then a running sum along the columns yields
+// the tile's WIN_TX window sums. A cell costs about 2 (WIN_TX + 2r) / WIN_TX
+// pixel terms instead of (2r+1)^2. Image rows (clamped to the edge, as the
+// reference's textures) sit in an LDS ring of 2r+2 rows, one word per pixel
+// so that neighbouring columns load in pairs; the next row is fetched into
+// registers while the current one is processed.
 //
 // Arithmetic is exact integer for SSD (uint32) and for every NCC window sum;
 // NCC combines them as
-//   zero-mean: rho = (N*Slr - Sl*Sr) / sqrt((N*Sll - Sl^2) (N*Srr - Sr^2))
-//   plain:     rho = Slr / sqrt(Sll * Srr)
-// with IEEE-rounded FP32 sqrt and division, cost = 1 - rho; a zero
-// denominator or x - d < 0 gives +inf.
+//   zero-mean: rho = (N*Slr - Sl*Sr) / (sqrt(N*Sll - Sl^2) * sqrt(N*Srr - Sr^2))
+//   plain:     rho = Slr / (sqrt(Sll) * sqrt(Srr))
+// (numerators exact in int32: |N*Slr|, Sl*Sr <= (121*255)^2 < 2^31), with
+// the per-pixel square roots IEEE-rounded once per pixel, the product of the
+// two roots IEEE-rounded, the quotient through the hardware reciprocal
+// (1 ulp) and cost = 1 - rho; a zero denominator or x - d < 0 gives +inf.
 #include "kernels.hpp"
 
 namespace tdvh {
 
 enum { WIN_SSD = 0, WIN_SSD_WTA = 1, WIN_NCC = 2 };
-constexpr int WIN_TY = 32; // output rows per block
+constexpr int WIN_TX = 16; // output columns per wave
+constexpr int WIN_TY = 32; // output rows per wave
 
 struct WinArgs {
     const uint8_t *L, *R;
@@ -28,117 +37,187 @@ struct WinArgs {
     int64_t sx, sy;     // output strides (elements)
     void *out;          // uint32 / float volume, or int16 disparities (WTA)
     int disp_pitch;
-    const int32_t *box; // NCC: 4 planes W*H: sum L, sum L^2, sum R, sum R^2
+    const int32_t *box; // NCC: 4 planes W*H: sum L, sqrt(var L) (f32), sum R, sqrt(var R) (f32)
     int zero_mean;
 };
 
 template <int RAD, int NCH, int MODE>
-__global__ __launch_bounds__(256) void window_kernel(const WinArgs a)
+__global__ __launch_bounds__(64) void window_kernel(const WinArgs a)
 {
     constexpr int NWIN = 2 * RAD + 1;
-    constexpr int DP = NCH * 64;
-    constexpr int RSPAN = DP + 3 + 2 * RAD; // staged right pixels per row
-    __shared__ uint8_t sL[4 + 2 * RAD];
-    __shared__ uint8_t sR[RSPAN];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int x0 = blockIdx.x * 4, x = x0 + wv;
-    const int y0 = blockIdx.y * WIN_TY;
+    constexpr int NC = WIN_TX + 2 * RAD;    // columns x0-RAD .. x0+WIN_TX-1+RAD
+    constexpr int RW = NC + NCH * 64 - 1;   // right pixels per staged row
+    constexpr int SLOTS = NWIN + 1;         // rows y-RAD-1 .. y+RAD
+    constexpr int ROW = NC + RW;            // words per slot: left, then right
+    constexpr int PER_LANE = (ROW + 63) / 64;
+    __shared__ uint32_t ring[SLOTS][ROW];
+    // NCC: the right image's box sums and roots of the output row, for right
+    // pixels x0 - (NCH*64 - 1) .. x0 + WIN_TX - 1
+    constexpr int BW = MODE == WIN_NCC ? WIN_TX + NCH * 64 - 1 : 1;
+    constexpr int B_PER_LANE = (BW + 63) / 64;
+    __shared__ int32_t sbr[BW];
+    __shared__ float sqr[BW];
+    __shared__ int32_t sbl[WIN_TX]; // left image, pixels x0 .. x0 + WIN_TX - 1
+    __shared__ float sql[WIN_TX];
+    const int lane = threadIdx.x;
+    const int x0 = blockIdx.x * WIN_TX, y0 = blockIdx.y * WIN_TY;
     const int W = a.W, H = a.H;
-    const int rbase = x0 - DP + 1 - RAD; // image x of sR[0]
+    const int y1 = min(y0 + WIN_TY, H); // one past the last output row
+    const int lbase = x0 - RAD, rbase = x0 - RAD - (NCH * 64 - 1);
 
-    uint32_t ring[NCH][NWIN];
-    uint32_t sum[NCH];
+    // fetch image row v (clamped) into registers: word k of the slot
+    uint32_t pre[PER_LANE];
+    auto fetch = [&](int v) {
+        const int vy = v < 0 ? 0 : (v >= H ? H - 1 : v);
+        const uint8_t *lr = a.L + (size_t)vy * a.pitch;
+        const uint8_t *rr = a.R + (size_t)vy * a.pitch;
 #pragma unroll
-    for (int c = 0; c < NCH; c++) {
-        sum[c] = 0;
-#pragma unroll
-        for (int k = 0; k < NWIN; k++) ring[c][k] = 0;
-    }
-    const int vend = min(y0 + WIN_TY, H) + RAD; // one past the last row staged
-    for (int vb = y0 - RAD; vb < vend; vb += NWIN) {
-#pragma unroll
-        for (int ph = 0; ph < NWIN; ph++) {
-            const int v = vb + ph;
-            if (v >= vend) break; // uniform across the block
-            const int vy = v < 0 ? 0 : (v >= H ? H - 1 : v);
-            const uint8_t *lr = a.L + (size_t)vy * a.pitch;
-            const uint8_t *rr = a.R + (size_t)vy * a.pitch;
-            __syncthreads(); // previous row's readers are done
-            for (int i = threadIdx.x; i < RSPAN; i += 256) {
-                const int u = rbase + i;
-                sR[i] = rr[u < 0 ? 0 : (u >= W ? W - 1 : u)];
+        for (int q = 0; q < PER_LANE; q++) {
+            const int k = lane + 64 * q;
+            uint32_t val = 0;
+            if (k < NC) {
+                const int u = lbase + k;
+                val = lr[u < 0 ? 0 : (u >= W ? W - 1 : u)];
+            } else if (k < ROW) {
+                const int u = rbase + (k - NC);
+                val = rr[u < 0 ? 0 : (u >= W ? W - 1 : u)];
             }
-            if (threadIdx.x < 4 + 2 * RAD) {
-                const int u = x0 - RAD + (int)threadIdx.x;
-                sL[threadIdx.x] = lr[u < 0 ? 0 : (u >= W ? W - 1 : u)];
+            pre[q] = val;
+        }
+    };
+    auto stash = [&](int slot) {
+#pragma unroll
+        for (int q = 0; q < PER_LANE; q++) {
+            const int k = lane + 64 * q;
+            if (k < ROW) ring[slot][k] = pre[q];
+        }
+    };
+    // pixel term of column i, this lane's disparity in chunk c, ring slot
+    auto term = [&](int slot, int i, int c) -> uint32_t {
+        const int lv = (int)ring[slot][i];
+        const int rv = (int)ring[slot][NC + i + NCH * 64 - 1 - (c * 64 + lane)];
+        if constexpr (MODE == WIN_NCC) return (uint32_t)(lv * rv);
+        else return (uint32_t)((lv - rv) * (lv - rv));
+    };
+
+    uint32_t col[NCH][NC];
+#pragma unroll
+    for (int c = 0; c < NCH; c++)
+#pragma unroll
+        for (int i = 0; i < NC; i++) col[c][i] = 0;
+
+    const int vfirst = y0 - RAD, vend = y1 + RAD; // entering rows
+    fetch(vfirst);
+    for (int v = vfirst; v < vend; v++) {
+        const int sin = (v - vfirst) % SLOTS;
+        __syncthreads(); // the slot's previous readers are done
+        stash(sin);
+        __syncthreads();
+        if (v + 1 < vend) fetch(v + 1);
+        // NCC: fetch the output row's right-image box values now, stage them
+        // after the term updates (their latency hides behind that work)
+        int32_t bpre[B_PER_LANE], lpre = 0;
+        uint32_t qpre[B_PER_LANE], lqpre = 0;
+        if constexpr (MODE == WIN_NCC) {
+            if (v - RAD >= y0) {
+                const size_t pl = (size_t)W * H, row = (size_t)(v - RAD) * W;
+                if (lane < WIN_TX) {
+                    const size_t o = row + min(x0 + lane, W - 1);
+                    lpre = a.box[o];
+                    lqpre = (uint32_t)a.box[pl + o];
+                }
+#pragma unroll
+                for (int q = 0; q < B_PER_LANE; q++) {
+                    const int k = lane + 64 * q;
+                    const int u = x0 - (NCH * 64 - 1) + k; // only 0 <= u < W are read later
+                    const size_t o = row + (u < 0 ? 0 : (u >= W ? W - 1 : u));
+                    bpre[q] = a.box[2 * pl + o];
+                    qpre[q] = (uint32_t)a.box[3 * pl + o];
+                }
+            }
+        }
+        const bool leave = v - vfirst >= NWIN;
+        const int sout = (v - vfirst - NWIN) % SLOTS;
+#pragma unroll
+        for (int c = 0; c < NCH; c++)
+#pragma unroll
+            for (int i = 0; i < NC; i++) {
+                uint32_t t = term(sin, i, c);
+                if (leave) t -= term(sout, i, c);
+                col[c][i] += t;
+            }
+        const int y = v - RAD; // output row whose window is complete
+        if (y < y0) continue;
+        if constexpr (MODE == WIN_NCC) {
+#pragma unroll
+            for (int q = 0; q < B_PER_LANE; q++) {
+                const int k = lane + 64 * q;
+                if (k < BW) {
+                    sbr[k] = bpre[q];
+                    sqr[k] = __builtin_bit_cast(float, qpre[q]);
+                }
+            }
+            if (lane < WIN_TX) {
+                sbl[lane] = lpre;
+                sql[lane] = __builtin_bit_cast(float, lqpre);
             }
             __syncthreads();
+        }
+        if constexpr (MODE == WIN_SSD_WTA) {
+            uint32_t key[WIN_TX];
+#pragma unroll
+            for (int t = 0; t < WIN_TX; t++) key[t] = 0xFFFFFFFFu;
 #pragma unroll
             for (int c = 0; c < NCH; c++) {
                 const int d = c * 64 + lane;
-                uint32_t t = 0;
+                uint32_t s = 0;
 #pragma unroll
-                for (int dx = -RAD; dx <= RAD; dx++) {
-                    const int lv = sL[wv + RAD + dx];
-                    // right pixel x + dx - d; sR already holds the clamped values
-                    const int rv = sR[x + dx - d - rbase];
-                    if (MODE == WIN_NCC)
-                        t += (uint32_t)(lv * rv);
-                    else
-                        t += (uint32_t)((lv - rv) * (lv - rv));
+                for (int i = 0; i < NWIN; i++) s += col[c][i];
+#pragma unroll
+                for (int t = 0; t < WIN_TX; t++) {
+                    if (t > 0) s += col[c][t + NWIN - 1] - col[c][t - 1];
+                    if (d < a.D && d <= x0 + t) key[t] = min(key[t], (s << 9) | (uint32_t)d);
                 }
-                sum[c] += t - ring[c][ph];
-                ring[c][ph] = t;
             }
-            const int y = v - RAD; // output row whose window is complete
-            if (y < y0 || x >= W) continue;
-            if (MODE == WIN_SSD_WTA) {
-                uint32_t key = 0xFFFFFFFFu;
 #pragma unroll
-                for (int c = 0; c < NCH; c++) {
-                    const int d = c * 64 + lane;
-                    if (d < a.D && d <= x) key = min(key, (sum[c] << 9) | (uint32_t)d);
-                }
+            for (int t = 0; t < WIN_TX; t++) {
                 // wave minimum (the 4 DPP row steps, then across the rows)
-                key = row_min_u32(key);
-                key = min(key, (uint32_t)__builtin_amdgcn_readlane((int)key, 0));
-                key = min(key, (uint32_t)__builtin_amdgcn_readlane((int)key, 16));
-                key = min(key, (uint32_t)__builtin_amdgcn_readlane((int)key, 32));
-                key = min(key, (uint32_t)__builtin_amdgcn_readlane((int)key, 48));
-                if (lane == 0)
-                    ((int16_t *)a.out)[(size_t)y * a.disp_pitch + x] =
-                        (int16_t)(key == 0xFFFFFFFFu ? 0 : (key & 511u));
-            } else {
+                uint32_t k = row_min_u32(key[t]);
+                k = min(k, (uint32_t)__builtin_amdgcn_readlane((int)k, 0));
+                k = min(k, (uint32_t)__builtin_amdgcn_readlane((int)k, 16));
+                k = min(k, (uint32_t)__builtin_amdgcn_readlane((int)k, 32));
+                k = min(k, (uint32_t)__builtin_amdgcn_readlane((int)k, 48));
+                if (lane == t && x0 + t < W)
+                    ((int16_t *)a.out)[(size_t)y * a.disp_pitch + x0 + t] =
+                        (int16_t)(k == 0xFFFFFFFFu ? 0 : (k & 511u));
+            }
+        } else {
 #pragma unroll
-                for (int c = 0; c < NCH; c++) {
-                    const int d = c * 64 + lane;
-                    if (d >= a.D) continue;
+            for (int c = 0; c < NCH; c++) {
+                const int d = c * 64 + lane;
+                uint32_t s = 0;
+#pragma unroll
+                for (int i = 0; i < NWIN; i++) s += col[c][i];
+#pragma unroll
+                for (int t = 0; t < WIN_TX; t++) {
+                    if (t > 0) s += col[c][t + NWIN - 1] - col[c][t - 1];
+                    const int x = x0 + t;
+                    if (x >= W || d >= a.D) continue;
                     const int64_t o = x * a.sx + y * a.sy + d;
-                    if (MODE == WIN_SSD) {
-                        ((uint32_t *)a.out)[o] = d <= x ? sum[c] : INV32;
+                    if constexpr (MODE == WIN_SSD) {
+                        ((uint32_t *)a.out)[o] = d <= x ? s : INV32;
                     } else {
                         float cost = __builtin_inff();
                         if (d <= x) {
-                            const size_t pl = (size_t)W * H;
-                            const size_t pli = (size_t)y * W + x, pri = pli - d;
-                            const int32_t sl = a.box[pli], sll = a.box[pl + pli];
-                            const int32_t sr = a.box[2 * pl + pri], srr = a.box[3 * pl + pri];
-                            int64_t num, vl, vr;
-                            if (a.zero_mean) {
-                                const int64_t N = (int64_t)NWIN * NWIN;
-                                num = N * (int64_t)sum[c] - (int64_t)sl * sr;
-                                vl = N * sll - (int64_t)sl * sl;
-                                vr = N * srr - (int64_t)sr * sr;
-                            } else {
-                                num = sum[c];
-                                vl = sll;
-                                vr = srr;
-                            }
-                            if (vl != 0 && vr != 0) {
-                                const float den = __fmul_rn(__fsqrt_rn((float)vl),
-                                                            __fsqrt_rn((float)vr));
-                                cost = __fsub_rn(1.0f, __fdiv_rn((float)num, den));
-                            }
+                            const int k = t + NCH * 64 - 1 - d; // right pixel x - d
+                            const int32_t sl = sbl[t], sr = sbr[k];
+                            const float ql = sql[t], qr = sqr[k];
+                            const int32_t num =
+                                a.zero_mean ? NWIN * NWIN * (int32_t)s - sl * sr : (int32_t)s;
+                            const float den = __fmul_rn(ql, qr);
+                            if (den != 0.0f)
+                                cost = __fsub_rn(1.0f,
+                                                 __fmul_rn((float)num, __builtin_amdgcn_rcpf(den)));
                         }
                         ((float *)a.out)[o] = cost;
                     }
@@ -148,11 +227,15 @@ __global__ __launch_bounds__(256) void window_kernel(const WinArgs a)
     }
 }
 
-// Per-pixel window sums of v and v^2 of both images (clamped windows).
+// Per-pixel window sums of both images (clamped windows) and the square
+// roots of their (zero-mean or plain) second moments:
+//   box = {sum L, sqrt(var L), sum R, sqrt(var R)}, var = N*sum v^2 - (sum v)^2
+// or sum v^2, exact in int64 and rounded once by the IEEE square root.
 template <int RAD>
 __global__ __launch_bounds__(256) void box_sums_kernel(const uint8_t *__restrict__ L,
                                                        const uint8_t *__restrict__ R, int W,
-                                                       int H, int pitch, int32_t *__restrict__ box)
+                                                       int H, int pitch, int zero_mean,
+                                                       int32_t *__restrict__ box)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= W) return;
@@ -168,24 +251,27 @@ __global__ __launch_bounds__(256) void box_sums_kernel(const uint8_t *__restrict
             srr += r * r;
         }
     }
+    const int64_t N = (int64_t)(2 * RAD + 1) * (2 * RAD + 1);
+    const int64_t vl = zero_mean ? N * sll - (int64_t)sl * sl : sll;
+    const int64_t vr = zero_mean ? N * srr - (int64_t)sr * sr : srr;
     const size_t pl = (size_t)W * H, o = (size_t)y * W + x;
     box[o] = sl;
-    box[pl + o] = sll;
+    box[pl + o] = __builtin_bit_cast(int32_t, __fsqrt_rn((float)vl));
     box[2 * pl + o] = sr;
-    box[3 * pl + o] = srr;
+    box[3 * pl + o] = __builtin_bit_cast(int32_t, __fsqrt_rn((float)vr));
 }
 
 template <int RAD, int MODE>
 static int launch_window_r(const WinArgs &a, hipStream_t s)
 {
-    dim3 grid((a.W + 3) / 4, (a.H + WIN_TY - 1) / WIN_TY);
+    dim3 grid((a.W + WIN_TX - 1) / WIN_TX, (a.H + WIN_TY - 1) / WIN_TY);
     const int nch = (a.D + 63) / 64;
     prof_begin(s);
     switch (nch) {
-    case 1: window_kernel<RAD, 1, MODE><<<grid, 256, 0, s>>>(a); break;
-    case 2: window_kernel<RAD, 2, MODE><<<grid, 256, 0, s>>>(a); break;
-    case 3: window_kernel<RAD, 3, MODE><<<grid, 256, 0, s>>>(a); break;
-    case 4: window_kernel<RAD, 4, MODE><<<grid, 256, 0, s>>>(a); break;
+    case 1: window_kernel<RAD, 1, MODE><<<grid, 64, 0, s>>>(a); break;
+    case 2: window_kernel<RAD, 2, MODE><<<grid, 64, 0, s>>>(a); break;
+    case 3: window_kernel<RAD, 3, MODE><<<grid, 64, 0, s>>>(a); break;
+    case 4: window_kernel<RAD, 4, MODE><<<grid, 64, 0, s>>>(a); break;
     default: set_error("window costs support D <= 256 (got %d)", a.D); return TDVH_ERR_UNSUPPORTED;
     }
     return check_launch(MODE == WIN_NCC ? "ncc_window_kernel"
@@ -266,7 +352,7 @@ extern "C" int tdvh_cost_ncc(tdvh_ctx *ctx, const uint8_t *left, const uint8_t *
     switch (radius) {
 #define CASE(r)                                                                                    \
     case r:                                                                                        \
-        box_sums_kernel<r><<<grid, 256, 0, s>>>(left, right, width, height, pitch,                 \
+        box_sums_kernel<r><<<grid, 256, 0, s>>>(left, right, width, height, pitch, zero_mean,      \
                                                  (int32_t *)box);                                  \
         break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5)
