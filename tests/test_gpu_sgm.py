@@ -133,9 +133,10 @@ def test_sgm_reference_geometry(tv, dev, W, H, D, flags):
 
 
 @pytest.mark.parametrize("dtype", ["u16", "u32", "i32", "f32"])
-def test_wta_lowest(tv, dev, dtype):
-    rng = np.random.default_rng(3)
-    H, W, D = 13, 17, 37
+@pytest.mark.parametrize("D", [37, 64, 200])  # D % 4 == 0: 4-cell vector loads per lane
+def test_wta_lowest(tv, dev, dtype, D):
+    rng = np.random.default_rng(3 + D)
+    H, W = 13, 17
     if dtype == "u16":
         S = rng.integers(0, 20, (H, W, D)).astype(np.uint16)
         S[rng.random(S.shape) < 0.1] = 0xFFFF

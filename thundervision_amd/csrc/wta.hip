@@ -48,6 +48,40 @@ __global__ __launch_bounds__(256) void wta_lowest_kernel(const T *__restrict__ S
     if (j == 0) disp[(size_t)y * pitch + x] = (int16_t)(md == 0xFFFFFFFFu ? 0 : md);
 }
 
+// The same scan with 4 consecutive cells per lane and load (D % 4 == 0 and
+// 4-element aligned pixels): lane j reads d = 4j .. 4j+3, then 64 further
+// on, as one streaming vector load each; within a lane d increases, so the
+// strict < keeps the lowest index, and the row reduction breaks value ties
+// by index.
+template <typename T>
+__global__ __launch_bounds__(256) void wta_lowest_vec_kernel(const T *__restrict__ S, int W,
+                                                             int H, int D, int64_t sx, int64_t sy,
+                                                             int16_t *__restrict__ disp,
+                                                             int pitch)
+{
+    typedef T vec4 __attribute__((ext_vector_type(4)));
+    const long p = (long)blockIdx.x * 16 + (threadIdx.x >> 4);
+    const int j = threadIdx.x & 15;
+    if (p >= (long)W * H) return;
+    const int y = (int)(p / W), x = (int)(p - (long)y * W);
+    const T *base = S + x * sx + y * sy;
+    uint32_t bv = 0xFFFFFFFFu, bd = 0xFFFFFFFFu;
+#pragma unroll 4
+    for (int d0 = 4 * j; d0 < D; d0 += 64) {
+        const vec4 v = __builtin_nontemporal_load((const vec4 *)(base + d0));
+        const uint32_t k[4] = {wta_key(v.x), wta_key(v.y), wta_key(v.z), wta_key(v.w)};
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+            if (k[e] < bv) {
+                bv = k[e];
+                bd = (uint32_t)(d0 + e);
+            }
+    }
+    const uint32_t mv = row_min_u32(bv);
+    const uint32_t md = row_min_u32(bv == mv ? bd : 0xFFFFFFFFu);
+    if (j == 0) disp[(size_t)y * pitch + x] = (int16_t)(md == 0xFFFFFFFFu ? 0 : md);
+}
+
 // One thread per pixel: tournament in bit-reversed order with a stack of
 // (value, index, subtree size).
 template <typename T>
@@ -90,6 +124,14 @@ static int launch_wta(const tdvh_dsi *S, int mode, int16_t *disp, int pitch, hip
     const long pixels = (long)S->width * S->height;
     if (mode == TDVH_WTA_LOWEST) {
         prof_begin(s);
+        const bool vec = S->depth % 4 == 0 && S->stride_x % 4 == 0 && S->stride_y % 4 == 0 &&
+                         ((uintptr_t)S->ptr & (4 * sizeof(T) - 1)) == 0;
+        if (vec) {
+            wta_lowest_vec_kernel<T><<<(unsigned)((pixels + 15) / 16), 256, 0, s>>>(
+                (const T *)S->ptr, S->width, S->height, S->depth, S->stride_x, S->stride_y, disp,
+                pitch);
+            return check_launch("wta_lowest_kernel", s);
+        }
         wta_lowest_kernel<T><<<(unsigned)((pixels + 15) / 16), 256, 0, s>>>(
             (const T *)S->ptr, S->width, S->height, S->depth, S->stride_x, S->stride_y, disp,
             pitch);
