@@ -15,14 +15,20 @@
 // neighbour of d = 0 and of d = 64K-1; cells d >= D carry invalid costs and
 // stay invalid, so no disparity bound needs a test). Costs stream through a
 // PF-step prefetch ring (one wave per SIMD leaves nothing else to hide the
-// load latency). Each step stores the lane's K two-bit decisions as one byte
-// (64 bytes per column). The backtrack stages 64 columns of decisions in LDS
-// at a time and walks them with a wave-uniform disparity.
+// load latency). Each step yields the lane's K two-bit decisions as one byte;
+// four columns' bytes are stored together as one word per lane (path word
+// (x/4)*64 + lane, byte x%4), so that stores, which share the memory counter
+// with the loads, take a quarter of its slots. The backtrack stages 64
+// columns of decisions in LDS at a time and walks them with a wave-uniform
+// disparity.
 #include "kernels.hpp"
 
 namespace tdvh {
 
 constexpr uint64_t DP_INF = ~0ull;
+
+// Bytes of one row's decisions: a word per lane for every 4 columns.
+__host__ __device__ inline size_t dp_row_bytes(int W) { return (size_t)((W + 3) / 4) * 256; }
 
 __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src)
 {
@@ -62,7 +68,7 @@ __global__ __launch_bounds__(64) void dp_kernel(const uint32_t *__restrict__ C, 
     const int l = threadIdx.x;
     const int y = blockIdx.x;
     const uint32_t *crow = C + y * sy;
-    uint8_t *prow = path + (size_t)y * W * 64;
+    uint32_t *prow = (uint32_t *)(path + (size_t)y * dp_row_bytes(W));
 
     auto load = [&](uint32_t(&c)[K], int x) {
         const int xx = x < W ? x : W - 1;
@@ -87,8 +93,10 @@ __global__ __launch_bounds__(64) void dp_kernel(const uint32_t *__restrict__ C, 
 #pragma unroll
     for (int u = 0; u < PF; u++) load(ring[u], 1 + u);
 
-    // one step at column x from ring slot u (which it then refills)
-    auto step = [&](int u, int x) {
+    // one step at column x from ring slot u (which it then refills); the
+    // decision byte goes into byte x%4 of cw, stored when the word is full
+    uint32_t cw = 0;
+    auto step = [&](int u, int x, bool tail) {
         uint32_t c[K];
 #pragma unroll
         for (int k = 0; k < K; k++) c[k] = ring[u][k];
@@ -121,18 +129,27 @@ __global__ __launch_bounds__(64) void dp_kernel(const uint32_t *__restrict__ C, 
         }
 #pragma unroll
         for (int k = 0; k < K; k++) A[k] = An[k];
-        prow[(size_t)x * 64 + l] = (uint8_t)codes;
+        cw |= codes << (8 * (x & 3));
+        if ((x & 3) == 3 || (tail && x == W - 1)) {
+            prow[(size_t)(x >> 2) * 64 + l] = cw;
+            cw = 0;
+        }
     };
     // whole blocks of PF steps without conditions (so the waits stay per
-    // slot), then the tail
+    // slot; PF % 4 == 0 and x0 % 4 == 1 make every x & 3 a constant), then
+    // the tail
+    static_assert(PF % 4 == 0, "whole decision words per block");
     int x0 = 1;
     for (; x0 + PF <= W; x0 += PF) {
 #pragma unroll
-        for (int u = 0; u < PF; u++) step(u, x0 + u);
+        for (int u = 0; u < PF; u++) step(u, x0 + u, false);
+    }
+    if (x0 == W) { // the last block ended the row inside a word
+        if (((W - 1) & 3) != 3) prow[(size_t)((W - 1) >> 2) * 64 + l] = cw;
     }
 #pragma unroll
     for (int u = 0; u < PF; u++)
-        if (x0 + u < W) step(u, x0 + u);
+        if (x0 + u < W) step(u, x0 + u, true);
 
     // start: lowest-index argmin of the last column (or d = 0)
     int start = 0;
@@ -164,7 +181,8 @@ __global__ __launch_bounds__(64) void dp_kernel(const uint32_t *__restrict__ C, 
         const int xlo = xc - 63 > 0 ? xc - 63 : 0;
         __syncthreads();
         for (int cx = xlo; cx <= xc; cx++)
-            sp[cx - xlo][l] = cx == 0 ? (uint8_t)0x55 : prow[(size_t)cx * 64 + l]; // x=0: "stay"
+            sp[cx - xlo][l] = cx == 0 ? (uint8_t)0x55 // x = 0: "stay"
+                                      : (uint8_t)(prow[(size_t)(cx >> 2) * 64 + l] >> (8 * (cx & 3)));
         __syncthreads();
         if (l == 0) {
             for (int x = xc; x >= xlo; x--) {
@@ -201,7 +219,7 @@ extern "C" int tdvh_dp(tdvh_ctx *ctx, const tdvh_dsi *C, uint32_t p_occ, int fla
                  flags);
     hipStream_t s = (hipStream_t)stream;
     void *pb = nullptr;
-    TDVH_TRY(ctx->scratch("dp_path", (size_t)W * H * 64, &pb));
+    TDVH_TRY(ctx->scratch("dp_path", dp_row_bytes(W) * H, &pb));
     const int K = (D + 63) / 64;
     prof_begin(s);
     const uint32_t *cp = (const uint32_t *)C->ptr;
