@@ -293,11 +293,14 @@ int tdvh_dsi_download(tdvh_ctx *ctx, const tdvh_dsi *dsi, void *host, void *stre
  * execution of DevStereoMatcher's WorkUnits on one CUDAProcess thread with a
  * blocking kernel sequence per stage (cudaprocess.cpp:9-30,
  * devstereomatcher.cpp:14-49) and the Dilate stage of Reconstruction
- * (reconstruction.cpp:30-31). tdvh_pipeline_run copies the images into the
- * pipeline's buffers, launches the graph and copies the disparities out, all
- * asynchronously on `stream`; the first run executes eagerly and captures the
- * graph. The pipeline owns a private context; runs on one pipeline must not
- * overlap (they share its buffers). */
+ * (reconstruction.cpp:30-31). tdvh_pipeline_run enqueues one frame
+ * asynchronously on `stream`: when the images and the disparity buffer are
+ * device memory the graph reads and writes them in place (a graph is captured
+ * for each new set of buffers and the last set is kept, so a serving loop over
+ * fixed buffers replays without copies); host buffers are copied into the
+ * pipeline's own device buffers and back around a second graph. The first run
+ * executes eagerly. The pipeline owns a private context; runs on one pipeline
+ * must not overlap (they share its buffers). */
 typedef struct tdvh_pipeline tdvh_pipeline;
 enum { TDVH_PIPE_PREFILTER = 1, TDVH_PIPE_POSTFILTER = 2 };
 int tdvh_pipeline_create(tdvh_ctx *ctx, int width, int height, int depth,

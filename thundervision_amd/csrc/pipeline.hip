@@ -4,12 +4,35 @@
 // The reference runs these as WorkUnits on one CUDAProcess thread, each stage
 // blocking on its own kernels (cudaprocess.cpp:9-30, devstereomatcher.cpp:14-49,
 // reconstruction.cpp:30-31). Here the stages are nodes of one graph on the
-// caller's stream: the host enqueues a frame with two copies and one graph
-// launch and returns. The pipeline owns a private tdvh context so that the
-// graph's fixed kernel arguments (scratch pointers) cannot be invalidated by
-// other calls; its first run executes eagerly (allocating scratch and building
-// per-shape tables) and then captures the graph, later runs replay it.
+// caller's stream: the host enqueues a frame as one graph launch and returns.
+// Device-resident images and disparity buffers are read and written in place
+// by a graph captured for that set of pointers (the last set is kept; a new
+// set is captured on its first run); host buffers go through the pipeline's
+// own device buffers with one copy each way around a second graph. The
+// pipeline owns a private tdvh context so that the graphs' fixed kernel
+// arguments (scratch pointers) cannot be invalidated by other calls; the
+// first run executes eagerly (allocating scratch and building per-shape
+// tables) and then captures.
 #include "kernels.hpp"
+
+// The buffers one graph reads and writes.
+struct PipeIO {
+    const uint8_t *L = nullptr, *R = nullptr;
+    int pitch = 0;
+    int16_t *disp = nullptr;
+    int disp_pitch = 0;
+    bool operator==(const PipeIO &o) const
+    {
+        return L == o.L && R == o.R && pitch == o.pitch && disp == o.disp &&
+               disp_pitch == o.disp_pitch;
+    }
+};
+
+struct PipeGraph {
+    PipeIO io;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
 
 struct tdvh_pipeline {
     tdvh_ctx *ctx = nullptr; // private context
@@ -19,33 +42,84 @@ struct tdvh_pipeline {
     int16_t *disp = nullptr;
     void *mem = nullptr;
     hipStream_t cap = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    bool warm = false;   // an eager run has allocated scratch and tables
+    PipeGraph staged;    // over the pipeline's own buffers (host images)
+    PipeGraph direct;    // over the caller's device buffers (last set seen)
 };
 
 namespace {
 
-// The stages of one frame on stream s (inputs in p->L, p->R, output p->disp).
-int pipeline_body(tdvh_pipeline *p, hipStream_t s)
+// The stages of one frame on stream s over the buffers io.
+int pipeline_body(tdvh_pipeline *p, const PipeIO &io, hipStream_t s)
 {
-    const uint8_t *l = p->L, *r = p->R;
+    const uint8_t *l = io.L, *r = io.R;
+    int pitch = io.pitch;
     if (p->flags & TDVH_PIPE_PREFILTER) {
-        TDVH_TRY(tdvh_prefilter_gauss5(p->ctx, p->L, p->W, p->W, p->H, p->Lf, p->W, s));
-        TDVH_TRY(tdvh_prefilter_gauss5(p->ctx, p->R, p->W, p->W, p->H, p->Rf, p->W, s));
+        TDVH_TRY(tdvh_prefilter_gauss5(p->ctx, io.L, io.pitch, p->W, p->H, p->Lf, p->W, s));
+        TDVH_TRY(tdvh_prefilter_gauss5(p->ctx, io.R, io.pitch, p->W, p->H, p->Rf, p->W, s));
         l = p->Lf;
         r = p->Rf;
+        pitch = p->W;
     }
-    TDVH_TRY(tdvh_stereo_bt_sgm(p->ctx, l, r, p->W, p->H, p->W, p->D, &p->params,
-                                TDVH_FUSE_COST, p->disp, p->W, s));
+    TDVH_TRY(tdvh_stereo_bt_sgm(p->ctx, l, r, p->W, p->H, pitch, p->D, &p->params,
+                                TDVH_FUSE_COST, io.disp, io.disp_pitch, s));
     if (p->flags & TDVH_PIPE_POSTFILTER)
-        TDVH_TRY(tdvh_postfilter_open(p->ctx, p->disp, p->W, p->W, p->H, 3, p->disp, p->W, s));
+        TDVH_TRY(tdvh_postfilter_open(p->ctx, io.disp, io.disp_pitch, p->W, p->H, 3, io.disp,
+                                      io.disp_pitch, s));
     return TDVH_OK;
+}
+
+void drop_graph(PipeGraph &g)
+{
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    g = PipeGraph{};
+}
+
+// Runs the frame over io on s from graph g. If g holds another buffer set it
+// is captured anew first; the pipeline's very first run executes eagerly (to
+// allocate scratch and tables) and captures the graph for the next run.
+int run_graph(tdvh_pipeline *p, PipeGraph &g, const PipeIO &io, hipStream_t s)
+{
+    bool launched = false;
+    if (!(g.exec && g.io == io)) {
+        drop_graph(g);
+        if (!p->warm) {
+            TDVH_TRY(pipeline_body(p, io, s));
+            p->warm = true;
+            launched = true;
+        }
+        if (!p->cap) TDVH_CHECK_HIP(hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking));
+        TDVH_CHECK_HIP(hipStreamSynchronize(s));
+        TDVH_CHECK_HIP(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
+        const int rc = pipeline_body(p, io, p->cap);
+        hipGraph_t gr = nullptr;
+        const hipError_t ec = hipStreamEndCapture(p->cap, &gr);
+        TDVH_TRY(rc);
+        TDVH_CHECK_HIP(ec);
+        g.graph = gr;
+        TDVH_CHECK_HIP(hipGraphInstantiate(&g.exec, gr, nullptr, nullptr, 0));
+        g.io = io;
+    }
+    if (!launched) TDVH_CHECK_HIP(hipGraphLaunch(g.exec, s));
+    return TDVH_OK;
+}
+
+// Whether ptr is device memory of the current device's address space.
+bool on_device(const void *ptr)
+{
+    hipPointerAttribute_t at{};
+    if (hipPointerGetAttributes(&at, ptr) != hipSuccess) {
+        (void)hipGetLastError(); // plain host memory
+        return false;
+    }
+    return at.type == hipMemoryTypeDevice;
 }
 
 void drop(tdvh_pipeline *p)
 {
-    if (p->exec) (void)hipGraphExecDestroy(p->exec);
-    if (p->graph) (void)hipGraphDestroy(p->graph);
+    drop_graph(p->staged);
+    drop_graph(p->direct);
     if (p->cap) (void)hipStreamDestroy(p->cap);
     if (p->mem) (void)hipFree(p->mem);
     if (p->ctx) (void)tdvh_ctx_destroy(p->ctx);
@@ -106,24 +180,24 @@ extern "C" int tdvh_pipeline_run(tdvh_pipeline *p, const uint8_t *left, const ui
                  pitch, disp_pitch, p->W);
     TDVH_TRY(use_device(p->ctx));
     hipStream_t s = (hipStream_t)stream;
+    if (on_device(left) && on_device(right) && on_device(disp)) {
+        PipeIO io;
+        io.L = left;
+        io.R = right;
+        io.pitch = pitch;
+        io.disp = disp;
+        io.disp_pitch = disp_pitch;
+        return run_graph(p, p->direct, io, s);
+    }
     TDVH_CHECK_HIP(hipMemcpy2DAsync(p->L, p->W, left, pitch, p->W, p->H, hipMemcpyDefault, s));
     TDVH_CHECK_HIP(hipMemcpy2DAsync(p->R, p->W, right, pitch, p->W, p->H, hipMemcpyDefault, s));
-    if (p->exec) {
-        TDVH_CHECK_HIP(hipGraphLaunch(p->exec, s));
-    } else {
-        // first frame: run eagerly (scratch, tables), then capture the stages
-        TDVH_TRY(pipeline_body(p, s));
-        if (!p->cap) TDVH_CHECK_HIP(hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking));
-        TDVH_CHECK_HIP(hipStreamSynchronize(s));
-        TDVH_CHECK_HIP(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
-        const int rc = pipeline_body(p, p->cap);
-        hipGraph_t g = nullptr;
-        const hipError_t ec = hipStreamEndCapture(p->cap, &g);
-        TDVH_TRY(rc);
-        TDVH_CHECK_HIP(ec);
-        p->graph = g;
-        TDVH_CHECK_HIP(hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0));
-    }
+    PipeIO io;
+    io.L = p->L;
+    io.R = p->R;
+    io.pitch = p->W;
+    io.disp = p->disp;
+    io.disp_pitch = p->W;
+    TDVH_TRY(run_graph(p, p->staged, io, s));
     TDVH_CHECK_HIP(hipMemcpy2DAsync(disp, (size_t)disp_pitch * 2, p->disp, (size_t)p->W * 2,
                                     (size_t)p->W * 2, p->H, hipMemcpyDefault, s));
     return TDVH_OK;
