@@ -54,3 +54,34 @@ def test_pipeline_720p_pitched(tv, dev):
             assert (out[:, W:] == -7).all()
     finally:
         p.close()
+
+
+def test_pipeline_host_and_device_buffers_interleaved(tv, dev):
+    """Pinned host buffers (copied in and out around the pipeline's own
+    graph) and device buffers (a graph over the caller's pointers, replayed
+    while the pointers stay the same) in alternation; new pixels in the same
+    device buffers between replays."""
+    W, H, D = 160, 96, 48
+    p = tv.Pipeline(W, H, D, flags=3)
+    Ld = torch.empty((H, W), dtype=torch.uint8, device=dev)
+    Rd = torch.empty_like(Ld)
+    outd = torch.empty((H, W), dtype=torch.int16, device=dev)
+    try:
+        for k in range(6):
+            L, R, _ = synth.pair(W, H, D - 1, 5, 70 + k)
+            if k % 2 == 0:
+                Lh = torch.from_numpy(L).pin_memory()
+                Rh = torch.from_numpy(R).pin_memory()
+                outh = torch.full((H, W), -1, dtype=torch.int16).pin_memory()
+                p.run(Lh, Rh, out=outh)
+                torch.cuda.synchronize()
+                got = outh.numpy()
+            else:
+                Ld.copy_(torch.from_numpy(L))
+                Rd.copy_(torch.from_numpy(R))
+                p.run(Ld, Rd, out=outd)
+                torch.cuda.synchronize()
+                got = outd.cpu().numpy()
+            assert (got == want(L, R, D, 3)).all(), k
+    finally:
+        p.close()
