@@ -126,19 +126,30 @@ __device__ __forceinline__ void sgm_step(uint32_t (&Lp)[M], const uint32_t (&c)[
     B = __builtin_amdgcn_perm(B, B, nb_sel_hi(j));
     // every L'(d) >= m, so t >= m and min(t, m + P2) - m = min(t - m, P2)
     const uint32_t mm = pk_splat(m), p2s = pk_splat(P2);
-    uint32_t nmin = 0xFFFFFFFFu;
+    // stage by stage over the M independent words, so that no packed op
+    // directly follows the op producing its operand (the words without a
+    // cross-lane neighbour first, while the row rotations complete)
+    uint32_t t[M];
 #pragma unroll
-    for (int r = 0; r < M; r++) {
+    for (int q = 0; q < M; q++) {
+        const int r = M < 3 ? q : (q < M - 2 ? q + 1 : (q == M - 2 ? 0 : M - 1));
         const uint32_t dm1 = r == 0 ? A : Lp[r - 1];
         const uint32_t dp1 = r == M - 1 ? B : Lp[r + 1];
-        const uint32_t t = pk_min(pk_adds(pk_min(dm1, dp1), p1), Lp[r]);
-        E[r] = pk_min(pk_sub(t, mm), p2s);
+        t[r] = pk_min(dm1, dp1);
     }
 #pragma unroll
-    for (int r = 0; r < M; r++) {
-        Lp[r] = pk_adds(E[r], c[r]);
-        nmin = pk_min(nmin, Lp[r]);
-    }
+    for (int r = 0; r < M; r++) t[r] = pk_adds(t[r], p1);
+#pragma unroll
+    for (int r = 0; r < M; r++) t[r] = pk_min(t[r], Lp[r]);
+#pragma unroll
+    for (int r = 0; r < M; r++) E[r] = pk_sub(t[r], mm);
+#pragma unroll
+    for (int r = 0; r < M; r++) E[r] = pk_min(E[r], p2s);
+#pragma unroll
+    for (int r = 0; r < M; r++) Lp[r] = pk_adds(E[r], c[r]);
+    uint32_t nmin = Lp[0];
+#pragma unroll
+    for (int r = 1; r < M; r++) nmin = pk_min(nmin, Lp[r]);
     m = row_min_u32(min(nmin & 0xFFFFu, nmin >> 16));
 }
 
@@ -414,13 +425,23 @@ __device__ __forceinline__ void sgm_line_group_buf(const SgmArgs &a, int dir, in
         if constexpr (FUSED) {
             uint32_t l2, lmn, lmx;
             lt_unpack(slot[0], l2, lmn, lmx);
+            // stage by stage over the words (independent ops between each
+            // packed op and its operands' producers)
+            uint32_t s1[M], s2[M], s3[M], s4[M];
 #pragma unroll
-            for (int r = 0; r < M; r++) {
-                const uint32_t r2 = slot[1 + r], rmn = slot[1 + M + r], rmx = slot[1 + 2 * M + r];
-                const uint32_t aa = pk_max(pk_subs(l2, rmx), pk_subs(rmn, l2));
-                const uint32_t bb = pk_max(pk_subs(r2, lmx), pk_subs(lmn, r2));
-                c[r] = pk_min(aa, bb);
-            }
+            for (int r = 0; r < M; r++) s1[r] = pk_subs(l2, slot[1 + 2 * M + r]);
+#pragma unroll
+            for (int r = 0; r < M; r++) s2[r] = pk_subs(slot[1 + M + r], l2);
+#pragma unroll
+            for (int r = 0; r < M; r++) s3[r] = pk_subs(slot[1 + r], lmx);
+#pragma unroll
+            for (int r = 0; r < M; r++) s4[r] = pk_subs(lmn, slot[1 + r]);
+#pragma unroll
+            for (int r = 0; r < M; r++) s1[r] = pk_max(s1[r], s2[r]);
+#pragma unroll
+            for (int r = 0; r < M; r++) s3[r] = pk_max(s3[r], s4[r]);
+#pragma unroll
+            for (int r = 0; r < M; r++) c[r] = pk_min(s1[r], s3[r]);
         } else {
 #pragma unroll
             for (int r = 0; r < M; r++) c[r] = slot[r];
