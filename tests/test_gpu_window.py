@@ -94,6 +94,22 @@ def test_dp(tv, dev, W, H, D, flags):
     assert (got.cpu().numpy() == want).all()
 
 
+@pytest.mark.parametrize("hi,inv", [(1 << 20, 0.0), (1 << 31, 0.05), (0xFFFFFFF0, 0.3)])
+def test_dp_arbitrary_costs(tv, dev, hi, inv):
+    """Costs far above SSD's range (the 32-bit pass gives up on such rows and
+    the 64-bit pass redoes them) and scattered invalid cells, both layouts of
+    the volume's strides."""
+    rng = np.random.default_rng(hi % 1000 + int(inv * 100))
+    H, W, D = 7, 300, 256
+    C = rng.integers(0, hi, (H, W, D), dtype=np.uint64).astype(np.uint32)
+    C[rng.random(C.shape) < inv] = 0xFFFFFFFF
+    want = oracle.dp(C, 20)
+    assert (tv.dp(to_dev(C.view(np.int32), dev), 20).cpu().numpy() == want).all()
+    # x-major storage ([x][y][d], the reference's DSI order) seen as (H, W, D)
+    Cx = to_dev(np.ascontiguousarray(C.transpose(1, 0, 2)).view(np.int32), dev).permute(1, 0, 2)
+    assert (tv.dp(Cx, 20).cpu().numpy() == want).all()
+
+
 def test_ssd_dp_window_1080p(tv, dev):
     """1920x1080, D = 256: 7x7 SSD (uint32) -> DP (P_occ = 20), bit-exact."""
     L, R, gt = synth.workload_pair(synth.WINDOW_1080P)
