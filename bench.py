@@ -208,6 +208,45 @@ def bench_window(tv, synth, torch, ctx, reps=5):
     return out
 
 
+def bench_plain_384(tv, synth, torch, ctx, with_cpu, reps=50):
+    """The 384x288, D = 16 pair (seed 1): the fused GPU call (images
+    resident), and with_cpu: the oracle's BT, SGM and WTA timed separately on
+    one host thread."""
+    spec = synth.PLAIN_384
+    W, H, D = spec["width"], spec["height"], spec["depth"]
+    L, R, _ = synth.workload_pair(spec)
+    dev = torch.device("cuda:0")
+    Ld, Rd = torch.from_numpy(L).to(dev), torch.from_numpy(R).to(dev)
+    out = torch.empty((H, W), dtype=torch.int16, device=dev)
+    fn = lambda: tv.stereo_bt_sgm(Ld, Rd, D, out=out, ctx=ctx)
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    cells = W * H * D
+    row = {"workload": "%dx%d uint8 pair, D=%d, seed 1; BT+SGM8+WTA" % (W, H, D),
+           "gpu_ms": round(ms, 4), "gpu_MDE/s": round(cells / (ms * 1e-3) / 1e6, 1)}
+    if with_cpu:
+        import oracle
+        t0 = time.perf_counter()
+        C = oracle.cost_bt2(L, R, D)
+        t1 = time.perf_counter()
+        S = oracle.sgm_std(C, 8, 32)
+        t2 = time.perf_counter()
+        d = oracle.wta(S)
+        t3 = time.perf_counter()
+        row["cpu_ms"] = {"bt": round((t1 - t0) * 1e3, 2), "sgm": round((t2 - t1) * 1e3, 2),
+                         "wta": round((t3 - t2) * 1e3, 2)}
+        row["cpu_MDE/s"] = round(cells / (t3 - t0) / 1e6, 2)
+        row["cpu"] = "oracle/tdvref.c gcc -O3, 1 thread (%s, nproc %d)" % (cpu_model(),
+                                                                           os.cpu_count() or 0)
+        row["bit_exact"] = bool((out.cpu().numpy() == d).all())
+    return row
+
+
 def bench_boundary(tv, torch, ctx, L, R, D, reps=20):
     """Steps around the path at the headline shape (device-resident unless
     stated): Gaussian prefilter, erode/dilate postfilter, colour to gray,
@@ -393,6 +432,9 @@ def main():
     boundary = bench_boundary(tv, torch, ctx, L, R, D)
     window = bench_window(tv, synth, torch, ctx)
 
+    # the 384x288, D = 16 pair (seed 1): GPU call, and the oracle per stage
+    plain_384 = bench_plain_384(tv, synth, torch, ctx, None if args.no_cpu_baseline else True)
+
     cpu = None
     if not args.no_cpu_baseline:
         import oracle
@@ -442,6 +484,7 @@ def main():
         "served": served,
         "boundary": boundary,
         "window_1080p": window,
+        "plain_384": plain_384,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
