@@ -207,3 +207,13 @@ def test_stereo_pitched_images(tv, dev):
     for fusion in (0, 1):
         got = tv.stereo_bt_sgm(big[0, :, :W], big[1, :, :W], D, fusion=fusion)
         assert (got.cpu().numpy() == want).all()
+
+
+@pytest.mark.parametrize("D", [64, 40])
+def test_stereo_1080p_two_word_lanes(tv, dev, D):
+    """1920x1080 at 2 words per lane: the line scans run 8-lane lines (8 per
+    wave, 4 words per lane); D = 40 adds padding cells."""
+    L, R, _ = synth.pair(1920, 1080, D - 1, 12, 7 + D)
+    want = oracle.stereo_bt_sgm(L, R, D)
+    got = tv.stereo_bt_sgm(to_dev(L, dev), to_dev(R, dev), D)
+    assert (got.cpu().numpy() == want).all()

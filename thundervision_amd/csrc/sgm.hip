@@ -57,10 +57,27 @@ static bool lines_buf_enabled()
 
 template <int M, bool FUSED>
 static void launch_lines_buf(const SgmArgs &a, const uint32_t *order, int groups, int prio,
-                             hipStream_t s)
+                             bool lanes8, hipStream_t s)
 {
-    if (a.D < 32 * M) sgm_lines_buf_kernel<M, FUSED, true><<<groups, 64, 0, s>>>(a, order, prio);
-    else sgm_lines_buf_kernel<M, FUSED, false><<<groups, 64, 0, s>>>(a, order, prio);
+    const bool pad = a.D < 32 * M;
+    if constexpr (FUSED && M == 2) {
+        if (lanes8) {
+            if (pad) sgm_lines_buf_kernel<M, FUSED, true, 8><<<groups, 64, 0, s>>>(a, order, prio);
+            else sgm_lines_buf_kernel<M, FUSED, false, 8><<<groups, 64, 0, s>>>(a, order, prio);
+            return;
+        }
+    }
+    if (pad) sgm_lines_buf_kernel<M, FUSED, true, 16><<<groups, 64, 0, s>>>(a, order, prio);
+    else sgm_lines_buf_kernel<M, FUSED, false, 16><<<groups, 64, 0, s>>>(a, order, prio);
+}
+
+// 8-lane lines (8 per wave, 4 words per lane) for the fused scans at 2 words
+// per lane on large images: measured 12 % faster at 1920x1080 (D = 64), but
+// slower where the launch is short and its longest lines set the time
+// (640x480: +3 %), and slower at 1 word per lane (1280x720, D = 32: +6 %).
+static bool lines_8lane(const SgmArgs &a)
+{
+    return a.C == nullptr && a.m == 2 && (long)a.W * a.H >= 1536L * 1024;
 }
 
 template <int M, bool FUSED, bool EOUT>
@@ -122,9 +139,10 @@ static int line_key(int dir, int sx, int sy)
 // launch runs on chiplet b % NXCD.
 constexpr int NXCD = 8;
 
-// Device table of the line groups (4 consecutive lines of one direction)
-// of every enabled direction. Entry: plane slot << 28 | direction << 24 |
-// first line; workgroup b runs entry b. Two goals:
+// Device table of the line groups (lpw consecutive lines of one direction:
+// the lines of one wave, 4, or 8 with 8-lane lines) of every enabled
+// direction. Entry: plane slot << 28 | direction << 24 | first line;
+// workgroup b runs entry b. Two goals:
 //  * longest first: with one wave per workgroup the dispatcher then runs a
 //    longest-processing-time-first list schedule;
 //  * chiplet locality: each direction's groups, in spatial order, are cut
@@ -133,7 +151,7 @@ constexpr int NXCD = 8;
 //    placement would make every chiplet fetch every row).
 // Entry b is the next entry of chiplet b % NXCD's longest-first queue.
 // Built and uploaded once per (W, H, dir_mask) and context.
-static int line_order(tdvh_ctx *ctx, int W, int H, int dir_mask, const uint32_t **out,
+static int line_order(tdvh_ctx *ctx, int W, int H, int dir_mask, int lpw, const uint32_t **out,
                       int *groups)
 {
     TDVH_REQUIRE(W + H < (1 << 24), "image %dx%d too large for the line table", W, H);
@@ -149,9 +167,9 @@ static int line_order(tdvh_ctx *ctx, int W, int H, int dir_mask, const uint32_t 
         const int n = dir_lines(dir, W, H);
         std::vector<Group> g;
         long work = 0;
-        for (int l0 = 0; l0 < n; l0 += 4) {
+        for (int l0 = 0; l0 < n; l0 += lpw) {
             int kmax = 0, key = 0;
-            for (int l = l0; l < l0 + 4 && l < n; l++) {
+            for (int l = l0; l < l0 + lpw && l < n; l++) {
                 int sx, sy, len;
                 line_geom(dir, l, W, H, sx, sy, len);
                 kmax = len > kmax ? len : kmax;
@@ -196,11 +214,12 @@ static int line_order(tdvh_ctx *ctx, int W, int H, int dir_mask, const uint32_t 
     // one table per shape and mask (a context serving several shapes keeps
     // them all, so steady-state calls never upload or synchronise)
     char name[64];
-    snprintf(name, sizeof name, "sgm_line_order:%dx%d:%02x", W, H, dir_mask);
+    snprintf(name, sizeof name, "sgm_line_order:%dx%d:%02x:%d", W, H, dir_mask, lpw);
     void *buf = nullptr;
     TDVH_TRY(ctx->scratch(name, h.size() * 4, &buf));
     tdvh_ctx::Slot &sl = ctx->slot(name);
-    const uint64_t tag = ((uint64_t)W << 32) | ((uint64_t)H << 8) | (uint64_t)dir_mask;
+    const uint64_t tag = ((uint64_t)W << 40) | ((uint64_t)H << 16) | ((uint64_t)lpw << 8) |
+                         (uint64_t)dir_mask;
     if (sl.tag != tag) {
         // queued launches may still read the previous table
         TDVH_CHECK_HIP(hipDeviceSynchronize());
@@ -219,14 +238,16 @@ int launch_sgm_lines(tdvh_ctx *ctx, const SgmArgs &a, int dir_mask, bool eout, h
     if (eout && lines_buf_enabled() && buf_lines_fit(a)) {
         const uint32_t *order = nullptr;
         int groups = 0;
-        TDVH_TRY(line_order(ctx, a.W, a.H, dir_mask, &order, &groups));
-        prof_begin(s);
         const bool fused = a.C == nullptr;
+        const bool lanes8 = lines_8lane(a);
+        const int lpw = lanes8 ? 8 : 4; // lines per wave
+        TDVH_TRY(line_order(ctx, a.W, a.H, dir_mask, lpw, &order, &groups));
+        prof_begin(s);
         switch (a.m) {
 #define CASE(m)                                                                                    \
     case m:                                                                                        \
-        if (fused) launch_lines_buf<m, true>(a, order, groups, ls.prio, s);                        \
-        else launch_lines_buf<m, false>(a, order, groups, ls.prio, s);                             \
+        if (fused) launch_lines_buf<m, true>(a, order, groups, ls.prio, lanes8, s);                \
+        else launch_lines_buf<m, false>(a, order, groups, ls.prio, false, s);                      \
         break;
             CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
 #undef CASE

@@ -65,11 +65,12 @@ __device__ __forceinline__ uint32_t pk_max(uint32_t a, uint32_t b)
     return u32(__builtin_elementwise_max(v2(a), v2(b)));
 }
 
-// Invalid-cell mask of lane j, word r: d >= D (padding) or d > x.
-template <int M>
+// Invalid-cell mask of lane j, word r: d >= D (padding) or d > x; M words
+// per lane, LPL lanes per pixel (word j*M + r holds d and d + LPL*M).
+template <int M, int LPL = 16>
 __device__ __forceinline__ uint32_t cell_mask(int j, int r, int D, int x)
 {
-    const int dlo = j * M + r, dhi = 16 * M + j * M + r;
+    const int dlo = j * M + r, dhi = LPL * M + j * M + r;
     return ((dlo >= D || dlo > x) ? 0x0000FFFFu : 0u) | ((dhi >= D || dhi > x) ? 0xFFFF0000u : 0u);
 }
 
@@ -102,6 +103,21 @@ __device__ __forceinline__ void bt_costs(const uint32_t (&lt)[3], const uint32_t
     }
 }
 
+// Min over aligned groups of G lanes (G = 2, 4, 8 or 16); every lane of a
+// group receives it.
+template <int G>
+__device__ __forceinline__ uint32_t group_min_u32(uint32_t v)
+{
+    if constexpr (G == 16) return row_min_u32(v);
+    if constexpr (G == 8) // lane i <-> 7-i inside each half row
+        v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false));
+    if constexpr (G >= 4) // quad_perm [2,3,0,1]
+        v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));
+    // quad_perm [1,0,3,2]
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));
+    return v;
+}
+
 // One SGM step on the packed vector of a line. Lp: previous L (in/out),
 // c: costs of the current pixel, m: previous minimum (in/out); E receives
 // L - C = min(...) - m, in [0, P2] for valid cells.
@@ -112,18 +128,36 @@ __device__ __forceinline__ void bt_costs(const uint32_t (&lt)[3], const uint32_t
 // of the last padded d is absent, i.e. (B >> 16) | 0xFFFF0000; other lanes
 // keep the rotated word.
 __device__ __forceinline__ uint32_t nb_sel_lo(int j) { return j == 0 ? 0x01000D0Du : 0x03020100u; }
-__device__ __forceinline__ uint32_t nb_sel_hi(int j) { return j == 15 ? 0x0D0D0302u : 0x03020100u; }
+template <int LPL = 16> __device__ __forceinline__ uint32_t nb_sel_hi(int j)
+{
+    return j == LPL - 1 ? 0x0D0D0302u : 0x03020100u;
+}
 
-template <int M>
+// LPL lanes per line (16, or 8 with 2M words per lane for M <= 2: the same
+// word order, so the same planes): j is the lane within the line's group.
+template <int M, int LPL = 16>
 __device__ __forceinline__ void sgm_step(uint32_t (&Lp)[M], const uint32_t (&c)[M], uint32_t &m,
                                          int j, uint32_t p1, uint32_t P2, uint32_t (&E)[M])
 {
+    static_assert(LPL == 16 || LPL == 8, "16 or 8 lanes per line");
     // row rotations write every lane of the row (no "old" operand), then one
     // per-lane byte permute applies the end-of-row fix-ups
-    uint32_t A = (uint32_t)__builtin_amdgcn_mov_dpp((int)Lp[M - 1], 0x121, 0xF, 0xF, false);
+    uint32_t A, B;
+    if constexpr (LPL == 16) {
+        A = (uint32_t)__builtin_amdgcn_mov_dpp((int)Lp[M - 1], 0x121, 0xF, 0xF, false);
+        B = (uint32_t)__builtin_amdgcn_mov_dpp((int)Lp[0], 0x12F, 0xF, 0xF, false);
+    } else {
+        // shifts by one inside the 16-lane row; the group's end lanes take the
+        // other end of their own 8-lane group (row_half_mirror: i <- 7 - i)
+        const uint32_t sr = (uint32_t)__builtin_amdgcn_mov_dpp((int)Lp[M - 1], 0x111, 0xF, 0xF, false);
+        const uint32_t mr = (uint32_t)__builtin_amdgcn_mov_dpp((int)Lp[M - 1], 0x141, 0xF, 0xF, false);
+        const uint32_t sl = (uint32_t)__builtin_amdgcn_mov_dpp((int)Lp[0], 0x101, 0xF, 0xF, false);
+        const uint32_t ml = (uint32_t)__builtin_amdgcn_mov_dpp((int)Lp[0], 0x141, 0xF, 0xF, false);
+        A = j == 0 ? mr : sr;
+        B = j == LPL - 1 ? ml : sl;
+    }
     A = __builtin_amdgcn_perm(A, A, nb_sel_lo(j));
-    uint32_t B = (uint32_t)__builtin_amdgcn_mov_dpp((int)Lp[0], 0x12F, 0xF, 0xF, false);
-    B = __builtin_amdgcn_perm(B, B, nb_sel_hi(j));
+    B = __builtin_amdgcn_perm(B, B, nb_sel_hi<LPL>(j));
     // every L'(d) >= m, so t >= m and min(t, m + P2) - m = min(t - m, P2)
     const uint32_t mm = pk_splat(m), p2s = pk_splat(P2);
     // stage by stage over the M independent words, so that no packed op
@@ -150,7 +184,7 @@ __device__ __forceinline__ void sgm_step(uint32_t (&Lp)[M], const uint32_t (&c)[
     uint32_t nmin = Lp[0];
 #pragma unroll
     for (int r = 1; r < M; r++) nmin = pk_min(nmin, Lp[r]);
-    m = row_min_u32(min(nmin & 0xFFFFu, nmin >> 16));
+    m = group_min_u32<LPL>(min(nmin & 0xFFFFu, nmin >> 16));
 }
 
 // Low bytes of the two halves of words a (-> bytes 0,1) and b (-> bytes 2,3).
@@ -352,21 +386,28 @@ __device__ __forceinline__ void buf_store(buf_t r, uint32_t off, const uint32_t 
     }
 }
 
-template <int M, bool FUSED, bool PAD, int PF>
+// LPL lanes per line: 16 (4 lines per wave, M words per lane), or 8 (8 lines
+// per wave, 2M words per lane; the planes and term rows are the same, only
+// the lanes owning each word differ). A smaller word count per lane leaves
+// the step's fixed costs (loads, stores, left terms, the group minimum)
+// spread over few cells; 8-lane lines halve the waves for the same work.
+template <int M, bool FUSED, bool PAD, int PF, int LPL = 16>
 __device__ __forceinline__ void sgm_line_group_buf(const SgmArgs &a, int dir, int line0, int s,
                                                    int prio)
 {
-    constexpr int NW = FUSED ? 1 + 3 * M : M; // prefetched words per step
+    constexpr int MW = M * 16 / LPL;           // words per lane and plane
+    constexpr int NW = FUSED ? 1 + 3 * MW : MW; // prefetched words per step
     const int lane = threadIdx.x & 63;
-    const int line = line0 + (lane >> 4);
-    const int j = lane & 15;
+    const int line = line0 + lane / LPL;
+    const int j = lane % LPL;
     const int W = a.W, H = a.H;
     int sx = 0, sy = 0, len = 0;
     if (line < dir_lines(dir, W, H)) line_geom(dir, line, W, H, sx, sy, len);
     const int dx = dir_dx(dir), dy = dir_dy(dir);
     const int step = dy * W + dx, p0 = sy * W + sx;
-    const int kmax = max(max(__builtin_amdgcn_readlane(len, 0), __builtin_amdgcn_readlane(len, 16)),
-                         max(__builtin_amdgcn_readlane(len, 32), __builtin_amdgcn_readlane(len, 48)));
+    int kmax = 0;
+#pragma unroll
+    for (int g = 0; g < 64 / LPL; g++) kmax = max(kmax, __builtin_amdgcn_readlane(len, g * LPL));
     if (prio) {
         const int longest = max(W, H);
         if (kmax * 4 >= longest * 3) __builtin_amdgcn_s_setprio(3);
@@ -375,7 +416,7 @@ __device__ __forceinline__ void sgm_line_group_buf(const SgmArgs &a, int dir, in
     }
 
     const buf_t out = buf_rsrc(a.planes + (size_t)s * a.plane_bytes, (uint32_t)a.plane_bytes);
-    uint32_t oE = (uint32_t)p0 * (32 * M) + j * (2 * M);
+    uint32_t oE = (uint32_t)p0 * (32 * M) + j * (2 * MW);
     const uint32_t dE = (uint32_t)(step * (32 * M));
 
     buf_t src[FUSED ? 4 : 1]; // packed left terms, then the three right-term planes
@@ -387,11 +428,11 @@ __device__ __forceinline__ void sgm_line_group_buf(const SgmArgs &a, int dir, in
         for (int t = 0; t < 3; t++) src[1 + t] = buf_rsrc(a.RT + (size_t)t * rt_plane, rt_plane * 4);
         fa = (uint32_t)p0 * 4;
         da = (uint32_t)(step * 4);
-        fb = (uint32_t)(sy * a.Wr + (W - 1 - sx) + j * M) * 4;
+        fb = (uint32_t)(sy * a.Wr + (W - 1 - sx) + j * MW) * 4;
         db = (uint32_t)((dy * a.Wr - dx) * 4);
     } else {
         src[0] = buf_rsrc(a.C, (uint32_t)W * H * (64 * M));
-        fa = (uint32_t)p0 * (64 * M) + j * (4 * M);
+        fa = (uint32_t)p0 * (64 * M) + j * (4 * MW);
         da = (uint32_t)(step * (64 * M));
     }
 
@@ -400,10 +441,10 @@ __device__ __forceinline__ void sgm_line_group_buf(const SgmArgs &a, int dir, in
         if constexpr (FUSED) {
             slot[0] = __builtin_amdgcn_raw_buffer_load_b32(src[0], fa, 0, 0);
 #pragma unroll
-            for (int t = 0; t < 3; t++) buf_load<M>(src[1 + t], fb, slot + 1 + t * M);
+            for (int t = 0; t < 3; t++) buf_load<MW>(src[1 + t], fb, slot + 1 + t * MW);
             fb += db;
         } else {
-            buf_load<M>(src[0], fa, slot);
+            buf_load<MW>(src[0], fa, slot);
         }
         fa += da;
     };
@@ -411,44 +452,44 @@ __device__ __forceinline__ void sgm_line_group_buf(const SgmArgs &a, int dir, in
     for (int u = 0; u < PF; u++) fetch(ring[u]);
 
     // L' = 0 and m = 0 make the first step return L = C (path start).
-    uint32_t Lp[M];
+    uint32_t Lp[MW];
 #pragma unroll
-    for (int r = 0; r < M; r++) Lp[r] = 0;
+    for (int r = 0; r < MW; r++) Lp[r] = 0;
     uint32_t m = 0;
     const uint32_t p1 = pk_splat(a.P1);
-    uint32_t padm[M];
+    uint32_t padm[MW];
 #pragma unroll
-    for (int r = 0; r < M; r++) padm[r] = cell_mask<M>(j, r, a.D, 1 << 30);
+    for (int r = 0; r < MW; r++) padm[r] = cell_mask<MW, LPL>(j, r, a.D, 1 << 30);
 
     // Costs of one step from a ring slot.
-    auto costs = [&](const uint32_t(&slot)[NW], uint32_t(&c)[M]) {
+    auto costs = [&](const uint32_t(&slot)[NW], uint32_t(&c)[MW]) {
         if constexpr (FUSED) {
             uint32_t l2, lmn, lmx;
             lt_unpack(slot[0], l2, lmn, lmx);
             // stage by stage over the words (independent ops between each
             // packed op and its operands' producers)
-            uint32_t s1[M], s2[M], s3[M], s4[M];
+            uint32_t s1[MW], s2[MW], s3[MW], s4[MW];
 #pragma unroll
-            for (int r = 0; r < M; r++) s1[r] = pk_subs(l2, slot[1 + 2 * M + r]);
+            for (int r = 0; r < MW; r++) s1[r] = pk_subs(l2, slot[1 + 2 * MW + r]);
 #pragma unroll
-            for (int r = 0; r < M; r++) s2[r] = pk_subs(slot[1 + M + r], l2);
+            for (int r = 0; r < MW; r++) s2[r] = pk_subs(slot[1 + MW + r], l2);
 #pragma unroll
-            for (int r = 0; r < M; r++) s3[r] = pk_subs(slot[1 + r], lmx);
+            for (int r = 0; r < MW; r++) s3[r] = pk_subs(slot[1 + r], lmx);
 #pragma unroll
-            for (int r = 0; r < M; r++) s4[r] = pk_subs(lmn, slot[1 + r]);
+            for (int r = 0; r < MW; r++) s4[r] = pk_subs(lmn, slot[1 + r]);
 #pragma unroll
-            for (int r = 0; r < M; r++) s1[r] = pk_max(s1[r], s2[r]);
+            for (int r = 0; r < MW; r++) s1[r] = pk_max(s1[r], s2[r]);
 #pragma unroll
-            for (int r = 0; r < M; r++) s3[r] = pk_max(s3[r], s4[r]);
+            for (int r = 0; r < MW; r++) s3[r] = pk_max(s3[r], s4[r]);
 #pragma unroll
-            for (int r = 0; r < M; r++) c[r] = pk_min(s1[r], s3[r]);
+            for (int r = 0; r < MW; r++) c[r] = pk_min(s1[r], s3[r]);
         } else {
 #pragma unroll
-            for (int r = 0; r < M; r++) c[r] = slot[r];
+            for (int r = 0; r < MW; r++) c[r] = slot[r];
         }
         if constexpr (PAD) {
 #pragma unroll
-            for (int r = 0; r < M; r++) c[r] |= padm[r];
+            for (int r = 0; r < MW; r++) c[r] |= padm[r];
         }
     };
 
@@ -458,26 +499,26 @@ __device__ __forceinline__ void sgm_line_group_buf(const SgmArgs &a, int dir, in
     // work the scheduler interleaves), then refills that slot. The scheduling
     // barrier keeps each body's waits on its own slot, so the compiler cannot
     // hoist later steps' cost math (and their waits) above earlier steps.
-    uint32_t c[M];
+    uint32_t c[MW];
     costs(ring[0], c);
     fetch(ring[0]);
     auto body = [&](int u, int k) {
-        uint32_t E[M];
-        sgm_step<M>(Lp, c, m, j, p1, (uint32_t)a.P2, E);
+        uint32_t E[MW];
+        sgm_step<MW, LPL>(Lp, c, m, j, p1, (uint32_t)a.P2, E);
         const uint32_t so = k < len ? oE : BUF_OOB;
         oE += dE;
-        if constexpr (M % 2 == 0) {
-            uint32_t w[M / 2];
+        if constexpr (MW % 2 == 0) {
+            uint32_t w[MW / 2];
 #pragma unroll
-            for (int r = 0; r < M; r += 2) w[r / 2] = pack_e(E[r], E[r + 1]);
-            if constexpr (M == 8) {
+            for (int r = 0; r < MW; r += 2) w[r / 2] = pack_e(E[r], E[r + 1]);
+            if constexpr (MW == 8) {
                 buf_store<4>(out, so, w);
             } else {
-                buf_store<M / 2>(out, so, w);
+                buf_store<MW / 2>(out, so, w);
             }
         } else {
 #pragma unroll
-            for (int r = 0; r < M; r++)
+            for (int r = 0; r < MW; r++)
                 __builtin_amdgcn_raw_buffer_store_b16((unsigned short)pack_e(E[r], 0u), out,
                                                       so + 2 * r, 0, BUF_STORE_NT);
         }
@@ -513,13 +554,13 @@ template <int M, bool FUSED> constexpr int buf_lines_pf()
 // One wave per workgroup, so the dispatcher starts the next line group as
 // soon as any wave ends; `order` lists the groups longest first (entry:
 // plane slot << 28 | direction << 24 | first line).
-template <int M, bool FUSED, bool PAD>
+template <int M, bool FUSED, bool PAD, int LPL>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void
 sgm_lines_buf_kernel(const SgmArgs a, const uint32_t *__restrict__ order, int prio)
 {
     const uint32_t e = order[blockIdx.x];
-    sgm_line_group_buf<M, FUSED, PAD, buf_lines_pf<M, FUSED>()>(a, (int)((e >> 24) & 0xF),
-                                                         (int)(e & 0xFFFFFF), (int)(e >> 28), prio);
+    sgm_line_group_buf<M, FUSED, PAD, buf_lines_pf<M, FUSED>(), LPL>(
+        a, (int)((e >> 24) & 0xF), (int)(e & 0xFFFFFF), (int)(e >> 28), prio);
 }
 
 // Whether sgm_line_group_buf can address every plane of a (byte planes).
@@ -674,20 +715,6 @@ __device__ __forceinline__ void sgm_sum_pixel(const SgmArgs &a, int nplanes,
     if (j == 0) disp[(size_t)y * pitch + x] = (int16_t)(key & 0xFFFFu);
 }
 
-// Min over aligned groups of G lanes (G = 2, 4, 8 or 16); every lane of a
-// group receives it.
-template <int G>
-__device__ __forceinline__ uint32_t group_min_u32(uint32_t v)
-{
-    if constexpr (G == 16) return row_min_u32(v);
-    if constexpr (G == 8) // lane i <-> 7-i inside each half row
-        v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false));
-    if constexpr (G >= 4) // quad_perm [2,3,0,1]
-        v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));
-    // quad_perm [1,0,3,2]
-    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));
-    return v;
-}
 
 // Word counts with a wide plane sum, and the lanes per pixel of that sum:
 // 2M (a DPP-aligned power of two), or 16 for M = 6 (12 working lanes).
