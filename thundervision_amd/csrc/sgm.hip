@@ -10,9 +10,10 @@
 // Mapping: one 16-lane DPP row owns one scan line and holds its disparity
 // vector in m packed uint16x2 registers per lane (SPLIT16 order: word j*m+r
 // of lane j is d = j*m+r in the low half and d = 16m + j*m+r in the high
-// half). A wave scans 4 lines; the d-1 / d+1 neighbours are register
-// renames except at the lane boundary (one row_ror DPP move each way plus a
-// fix-up on the end lanes), and m is a 4-step DPP row all-reduce. Every
+// half). A wave scans 4 lines (8 lines of 8 lanes with 2m words each for
+// m = 2 on large images); the d-1 / d+1 neighbours are register renames
+// except at the lane boundary (one DPP move each way plus a byte-permute
+// fix-up on the end lanes), and m is a DPP all-reduce over the line's lanes. Every
 // direction's lines are independent, so all 8 directions run in one launch
 // and each writes its own path-cost plane; a second kernel sums the planes
 // and takes the lowest-index argmin (wtacpu.cpp:20-33).

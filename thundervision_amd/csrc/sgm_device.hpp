@@ -552,8 +552,9 @@ template <int M, bool FUSED> constexpr int buf_lines_pf()
 }
 
 // One wave per workgroup, so the dispatcher starts the next line group as
-// soon as any wave ends; `order` lists the groups longest first (entry:
-// plane slot << 28 | direction << 24 | first line).
+// soon as any wave ends; `order` gives workgroup b its group (entry: plane
+// slot << 28 | direction << 24 | first line; 64 / LPL lines from there),
+// longest first within each chiplet's share.
 template <int M, bool FUSED, bool PAD, int LPL>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void
 sgm_lines_buf_kernel(const SgmArgs a, const uint32_t *__restrict__ order, int prio)
