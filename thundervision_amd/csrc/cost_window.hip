@@ -93,11 +93,10 @@ __global__ __launch_bounds__(64) void window_kernel(const WinArgs a)
         }
     };
     // pixel term of column i, this lane's disparity in chunk c, ring slot
-    auto term = [&](int slot, int i, int c) -> uint32_t {
-        const int lv = (int)ring[slot][i];
-        const int rv = (int)ring[slot][NC + i + NCH * 64 - 1 - (c * 64 + lane)];
-        if constexpr (MODE == WIN_NCC) return (uint32_t)(lv * rv);
-        else return (uint32_t)((lv - rv) * (lv - rv));
+    // pixel term of a left and a right value (bytes, so 24-bit products)
+    auto term = [](int lv, int rv) -> uint32_t {
+        if constexpr (MODE == WIN_NCC) return (uint32_t)__mul24(lv, rv);
+        else return (uint32_t)__mul24(lv - rv, lv - rv);
     };
 
     uint32_t col[NCH][NC];
@@ -138,14 +137,31 @@ __global__ __launch_bounds__(64) void window_kernel(const WinArgs a)
         }
         const bool leave = v - vfirst >= NWIN;
         const int sout = (v - vfirst - NWIN) % SLOTS;
+        // the entering (and leaving) row's left values, then per chunk the
+        // lane's right values, read as whole rows before the arithmetic
+        int lin[NC], lout[NC];
 #pragma unroll
-        for (int c = 0; c < NCH; c++)
+        for (int i = 0; i < NC; i++) lin[i] = (int)ring[sin][i];
+        if (leave) {
 #pragma unroll
-            for (int i = 0; i < NC; i++) {
-                uint32_t t = term(sin, i, c);
-                if (leave) t -= term(sout, i, c);
-                col[c][i] += t;
+            for (int i = 0; i < NC; i++) lout[i] = (int)ring[sout][i];
+        }
+#pragma unroll
+        for (int c = 0; c < NCH; c++) {
+            const int rb = NC + NCH * 64 - 1 - (c * 64 + lane); // right value of column 0
+            int rin[NC];
+#pragma unroll
+            for (int i = 0; i < NC; i++) rin[i] = (int)ring[sin][rb + i];
+#pragma unroll
+            for (int i = 0; i < NC; i++) col[c][i] += term(lin[i], rin[i]);
+            if (leave) {
+                int rout[NC];
+#pragma unroll
+                for (int i = 0; i < NC; i++) rout[i] = (int)ring[sout][rb + i];
+#pragma unroll
+                for (int i = 0; i < NC; i++) col[c][i] -= term(lout[i], rout[i]);
             }
+        }
         const int y = v - RAD; // output row whose window is complete
         if (y < y0) continue;
         if constexpr (MODE == WIN_NCC) {
