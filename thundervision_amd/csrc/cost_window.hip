@@ -208,34 +208,41 @@ __global__ __launch_bounds__(64) void window_kernel(const WinArgs a)
                         (int16_t)(k == 0xFFFFFFFFu ? 0 : (k & 511u));
             }
         } else {
+            // one store per column and chunk: lanes with d >= D skip their
+            // chunk, columns past the image end the row (both uniform tests)
+            const int tn = min(WIN_TX, W - x0);
 #pragma unroll
             for (int c = 0; c < NCH; c++) {
                 const int d = c * 64 + lane;
+                if (d >= a.D) continue;
                 uint32_t s = 0;
 #pragma unroll
                 for (int i = 0; i < NWIN; i++) s += col[c][i];
+                char *p = (char *)a.out + ((int64_t)x0 * a.sx + (int64_t)y * a.sy + d) * 4;
+                const int64_t px = a.sx * 4;
 #pragma unroll
                 for (int t = 0; t < WIN_TX; t++) {
                     if (t > 0) s += col[c][t + NWIN - 1] - col[c][t - 1];
+                    if (t >= tn) break;
                     const int x = x0 + t;
-                    if (x >= W || d >= a.D) continue;
-                    const int64_t o = x * a.sx + y * a.sy + d;
                     if constexpr (MODE == WIN_SSD) {
-                        ((uint32_t *)a.out)[o] = d <= x ? s : INV32;
+                        *(uint32_t *)(p + t * px) = d <= x ? s : INV32;
                     } else {
-                        float cost = __builtin_inff();
-                        if (d <= x) {
-                            const int k = t + NCH * 64 - 1 - d; // right pixel x - d
-                            const int32_t sl = sbl[t], sr = sbr[k];
-                            const float ql = sql[t], qr = sqr[k];
-                            const int32_t num =
-                                a.zero_mean ? NWIN * NWIN * (int32_t)s - sl * sr : (int32_t)s;
-                            const float den = __fmul_rn(ql, qr);
-                            if (den != 0.0f)
-                                cost = __fsub_rn(1.0f,
-                                                 __fmul_rn((float)num, __builtin_amdgcn_rcpf(den)));
-                        }
-                        ((float *)a.out)[o] = cost;
+                        // the terms of x - d < 0 are clamped pixels: computed,
+                        // then replaced by the invalid cost
+                        const int k = t + NCH * 64 - 1 - d; // right pixel x - d
+                        const int32_t sl = sbl[t], sr = sbr[k];
+                        const float ql = sql[t], qr = sqr[k];
+                        // 24-bit operands: s < 121 * 65025 < 2^23, sl, sr < 2^15
+                        const int32_t num = a.zero_mean ? __mul24(NWIN * NWIN, (int32_t)s) -
+                                                              __mul24(sl, sr)
+                                                        : (int32_t)s;
+                        const float den = __fmul_rn(ql, qr);
+                        const float cost =
+                            (d <= x && den != 0.0f)
+                                ? __fsub_rn(1.0f, __fmul_rn((float)num, __builtin_amdgcn_rcpf(den)))
+                                : __builtin_inff();
+                        *(float *)(p + t * px) = cost;
                     }
                 }
             }
