@@ -20,9 +20,10 @@
 //   zero-mean: rho = (N*Slr - Sl*Sr) / (sqrt(N*Sll - Sl^2) * sqrt(N*Srr - Sr^2))
 //   plain:     rho = Slr / (sqrt(Sll) * sqrt(Srr))
 // (numerators exact in int32: |N*Slr|, Sl*Sr <= (121*255)^2 < 2^31), with
-// the per-pixel square roots IEEE-rounded once per pixel, the product of the
-// two roots IEEE-rounded, the quotient through the hardware reciprocal
-// (1 ulp) and cost = 1 - rho; a zero denominator or x - d < 0 gives +inf.
+// the per-pixel reciprocal square roots IEEE-rounded (root, then quotient)
+// once per pixel, their product IEEE-rounded, and cost = 1 - num * product in
+// one fused multiply-add; a zero variance (an infinite reciprocal root) or
+// x - d < 0 gives +inf.
 #include "kernels.hpp"
 
 namespace tdvh {
@@ -37,7 +38,7 @@ struct WinArgs {
     int64_t sx, sy;     // output strides (elements)
     void *out;          // uint32 / float volume, or int16 disparities (WTA)
     int disp_pitch;
-    const int32_t *box; // NCC: 4 planes W*H: sum L, sqrt(var L) (f32), sum R, sqrt(var R) (f32)
+    const int32_t *box; // NCC: 4 planes W*H: sum L, 1/sqrt(var L) (f32), sum R, 1/sqrt(var R) (f32)
     int zero_mean;
 };
 
@@ -51,7 +52,7 @@ __global__ __launch_bounds__(64) void window_kernel(const WinArgs a)
     constexpr int ROW = NC + RW;            // words per slot: left, then right
     constexpr int PER_LANE = (ROW + 63) / 64;
     __shared__ uint32_t ring[SLOTS][ROW];
-    // NCC: the right image's box sums and roots of the output row, for right
+    // NCC: the right image's box sums and reciprocal roots of the output row, for right
     // pixels x0 - (NCH*64 - 1) .. x0 + WIN_TX - 1
     constexpr int BW = MODE == WIN_NCC ? WIN_TX + NCH * 64 - 1 : 1;
     constexpr int B_PER_LANE = (BW + 63) / 64;
@@ -237,11 +238,11 @@ __global__ __launch_bounds__(64) void window_kernel(const WinArgs a)
                         const int32_t num = a.zero_mean ? __mul24(NWIN * NWIN, (int32_t)s) -
                                                               __mul24(sl, sr)
                                                         : (int32_t)s;
-                        const float den = __fmul_rn(ql, qr);
-                        const float cost =
-                            (d <= x && den != 0.0f)
-                                ? __fsub_rn(1.0f, __fmul_rn((float)num, __builtin_amdgcn_rcpf(den)))
-                                : __builtin_inff();
+                        // ql, qr: reciprocal roots (+inf for a zero variance)
+                        const float inv = __fmul_rn(ql, qr);
+                        const float cost = (d <= x && inv != __builtin_inff())
+                                               ? __fmaf_rn(-(float)num, inv, 1.0f)
+                                               : __builtin_inff();
                         *(float *)(p + t * px) = cost;
                     }
                 }
@@ -252,8 +253,9 @@ __global__ __launch_bounds__(64) void window_kernel(const WinArgs a)
 
 // Per-pixel window sums of both images (clamped windows) and the square
 // roots of their (zero-mean or plain) second moments:
-//   box = {sum L, sqrt(var L), sum R, sqrt(var R)}, var = N*sum v^2 - (sum v)^2
-// or sum v^2, exact in int64 and rounded once by the IEEE square root.
+//   box = {sum L, 1/sqrt(var L), sum R, 1/sqrt(var R)}, var = N*sum v^2 - (sum v)^2
+// or sum v^2, exact in int64, then the IEEE square root and the IEEE quotient
+// (a zero variance gives +inf).
 template <int RAD>
 __global__ __launch_bounds__(256) void box_sums_kernel(const uint8_t *__restrict__ L,
                                                        const uint8_t *__restrict__ R, int W,
@@ -279,9 +281,9 @@ __global__ __launch_bounds__(256) void box_sums_kernel(const uint8_t *__restrict
     const int64_t vr = zero_mean ? N * srr - (int64_t)sr * sr : srr;
     const size_t pl = (size_t)W * H, o = (size_t)y * W + x;
     box[o] = sl;
-    box[pl + o] = __builtin_bit_cast(int32_t, __fsqrt_rn((float)vl));
+    box[pl + o] = __builtin_bit_cast(int32_t, __fdiv_rn(1.0f, __fsqrt_rn((float)vl)));
     box[2 * pl + o] = sr;
-    box[3 * pl + o] = __builtin_bit_cast(int32_t, __fsqrt_rn((float)vr));
+    box[3 * pl + o] = __builtin_bit_cast(int32_t, __fdiv_rn(1.0f, __fsqrt_rn((float)vr)));
 }
 
 template <int RAD, int MODE>
