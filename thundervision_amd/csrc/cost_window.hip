@@ -58,8 +58,6 @@ __global__ __launch_bounds__(64) void window_kernel(const WinArgs a)
     constexpr int B_PER_LANE = (BW + 63) / 64;
     __shared__ int32_t sbr[BW];
     __shared__ float sqr[BW];
-    __shared__ int32_t sbl[WIN_TX]; // left image, pixels x0 .. x0 + WIN_TX - 1
-    __shared__ float sql[WIN_TX];
     const int lane = threadIdx.x;
     const int x0 = blockIdx.x * WIN_TX, y0 = blockIdx.y * WIN_TY;
     const int W = a.W, H = a.H;
@@ -132,7 +130,9 @@ __global__ __launch_bounds__(64) void window_kernel(const WinArgs a)
                     const int u = x0 - (NCH * 64 - 1) + k; // only 0 <= u < W are read later
                     const size_t o = row + (u < 0 ? 0 : (u >= W ? W - 1 : u));
                     bpre[q] = a.box[2 * pl + o];
-                    qpre[q] = (uint32_t)a.box[3 * pl + o];
+                    // x - d < 0: an infinite reciprocal root makes the cost
+                    // invalid
+                    qpre[q] = u < 0 ? 0x7F800000u : (uint32_t)a.box[3 * pl + o];
                 }
             }
         }
@@ -174,10 +174,6 @@ __global__ __launch_bounds__(64) void window_kernel(const WinArgs a)
                     sqr[k] = __builtin_bit_cast(float, qpre[q]);
                 }
             }
-            if (lane < WIN_TX) {
-                sbl[lane] = lpre;
-                sql[lane] = __builtin_bit_cast(float, lqpre);
-            }
             __syncthreads();
         }
         if constexpr (MODE == WIN_SSD_WTA) {
@@ -208,6 +204,50 @@ __global__ __launch_bounds__(64) void window_kernel(const WinArgs a)
                     ((int16_t *)a.out)[(size_t)y * a.disp_pitch + x0 + t] =
                         (int16_t)(k == 0xFFFFFFFFu ? 0 : (k & 511u));
             }
+        } else if constexpr (MODE == WIN_NCC) {
+            // columns outer, chunks inner: a column's four right-value reads
+            // go out together, and the left pixel's values (lane t holds
+            // pixel x0 + t) come from registers by readlane
+            const int tn = min(WIN_TX, W - x0);
+            // plain NCC: numerator = Slr (the left sum is taken as 0)
+            const int32_t mult = a.zero_mean ? NWIN * NWIN : 1;
+            uint32_t sum[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; c++) {
+                sum[c] = 0;
+#pragma unroll
+                for (int i = 0; i < NWIN; i++) sum[c] += col[c][i];
+            }
+            char *p = (char *)a.out + ((int64_t)x0 * a.sx + (int64_t)y * a.sy + lane) * 4;
+            const int64_t px = a.sx * 4;
+#pragma unroll
+            for (int t = 0; t < WIN_TX; t++) {
+                if (t > 0) {
+#pragma unroll
+                    for (int c = 0; c < NCH; c++) sum[c] += col[c][t + NWIN - 1] - col[c][t - 1];
+                }
+                if (t >= tn) break;
+                const int32_t sl = a.zero_mean ? __builtin_amdgcn_readlane(lpre, t) : 0;
+                const float ql =
+                    __builtin_bit_cast(float, __builtin_amdgcn_readlane((int)lqpre, t));
+#pragma unroll
+                for (int c = 0; c < NCH; c++) {
+                    const int d = c * 64 + lane;
+                    // the right pixel x - d (for x - d < 0 a clamped pixel
+                    // whose staged reciprocal root is +inf)
+                    const int k = t + NCH * 64 - 1 - d;
+                    const int32_t sr = sbr[k];
+                    const float qr = sqr[k];
+                    // 24-bit operands: sum < 121 * 65025 < 2^23, sl, sr < 2^15;
+                    // ql, qr: reciprocal roots (+inf for a zero variance)
+                    const int32_t num = __mul24(mult, (int32_t)sum[c]) - __mul24(sl, sr);
+                    const float inv = __fmul_rn(ql, qr);
+                    const float cost = inv != __builtin_inff() ? __fmaf_rn(-(float)num, inv, 1.0f)
+                                                               : __builtin_inff();
+                    // only the last chunk can hold d >= D
+                    if (c < NCH - 1 || d < a.D) *(float *)(p + t * px + c * 256) = cost;
+                }
+            }
         } else {
             // one store per column and chunk: lanes with d >= D skip their
             // chunk, columns past the image end the row (both uniform tests)
@@ -226,25 +266,7 @@ __global__ __launch_bounds__(64) void window_kernel(const WinArgs a)
                     if (t > 0) s += col[c][t + NWIN - 1] - col[c][t - 1];
                     if (t >= tn) break;
                     const int x = x0 + t;
-                    if constexpr (MODE == WIN_SSD) {
-                        *(uint32_t *)(p + t * px) = d <= x ? s : INV32;
-                    } else {
-                        // the terms of x - d < 0 are clamped pixels: computed,
-                        // then replaced by the invalid cost
-                        const int k = t + NCH * 64 - 1 - d; // right pixel x - d
-                        const int32_t sl = sbl[t], sr = sbr[k];
-                        const float ql = sql[t], qr = sqr[k];
-                        // 24-bit operands: s < 121 * 65025 < 2^23, sl, sr < 2^15
-                        const int32_t num = a.zero_mean ? __mul24(NWIN * NWIN, (int32_t)s) -
-                                                              __mul24(sl, sr)
-                                                        : (int32_t)s;
-                        // ql, qr: reciprocal roots (+inf for a zero variance)
-                        const float inv = __fmul_rn(ql, qr);
-                        const float cost = (d <= x && inv != __builtin_inff())
-                                               ? __fmaf_rn(-(float)num, inv, 1.0f)
-                                               : __builtin_inff();
-                        *(float *)(p + t * px) = cost;
-                    }
+                    *(uint32_t *)(p + t * px) = d <= x ? s : INV32;
                 }
             }
         }
