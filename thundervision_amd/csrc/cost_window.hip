@@ -249,24 +249,37 @@ __global__ __launch_bounds__(64) void window_kernel(const WinArgs a)
                 }
             }
         } else {
-            // one store per column and chunk: lanes with d >= D skip their
-            // chunk, columns past the image end the row (both uniform tests)
+            // columns outer, chunks inner (a pixel's chunks are one
+            // contiguous run of 4 * 64 costs); columns past the image end the
+            // row (a uniform test)
             const int tn = min(WIN_TX, W - x0);
+            uint32_t sum[NCH];
 #pragma unroll
             for (int c = 0; c < NCH; c++) {
-                const int d = c * 64 + lane;
-                if (d >= a.D) continue;
-                uint32_t s = 0;
+                sum[c] = 0;
 #pragma unroll
-                for (int i = 0; i < NWIN; i++) s += col[c][i];
-                char *p = (char *)a.out + ((int64_t)x0 * a.sx + (int64_t)y * a.sy + d) * 4;
-                const int64_t px = a.sx * 4;
+                for (int i = 0; i < NWIN; i++) sum[c] += col[c][i];
+            }
+            char *p = (char *)a.out + ((int64_t)x0 * a.sx + (int64_t)y * a.sy + lane) * 4;
+            const int64_t px = a.sx * 4;
+            // x - d = rem + t - 64 c; hidden from the compiler per row, which
+            // would otherwise hoist every (t, c) comparison out of the row
+            // loop into spilled scalar masks
+            int rem = x0 - lane;
+            asm volatile("" : "+v"(rem));
 #pragma unroll
-                for (int t = 0; t < WIN_TX; t++) {
-                    if (t > 0) s += col[c][t + NWIN - 1] - col[c][t - 1];
-                    if (t >= tn) break;
-                    const int x = x0 + t;
-                    *(uint32_t *)(p + t * px) = d <= x ? s : INV32;
+            for (int t = 0; t < WIN_TX; t++) {
+                if (t > 0) {
+#pragma unroll
+                    for (int c = 0; c < NCH; c++) sum[c] += col[c][t + NWIN - 1] - col[c][t - 1];
+                }
+                if (t >= tn) break;
+#pragma unroll
+                for (int c = 0; c < NCH; c++) {
+                    const uint32_t v = rem + t - 64 * c >= 0 ? sum[c] : INV32;
+                    // only the last chunk can hold d >= D
+                    if (c < NCH - 1 || c * 64 + lane < a.D)
+                        *(uint32_t *)(p + t * px + c * 256) = v;
                 }
             }
         }
