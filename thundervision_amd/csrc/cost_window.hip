@@ -180,6 +180,13 @@ __global__ __launch_bounds__(64) void window_kernel(const WinArgs a)
             uint32_t key[WIN_TX];
 #pragma unroll
             for (int t = 0; t < WIN_TX; t++) key[t] = 0xFFFFFFFFu;
+            // an invalid cell's key is all ones (above every valid key: s < 2^22):
+            // x - d = rem + t - 64 c < 0 sets it through the sign, and so does
+            // d >= D in the last chunk; rem is hidden from the compiler per
+            // row so that nothing is hoisted out of the row loop
+            int rem = x0 - lane;
+            asm volatile("" : "+v"(rem));
+            const uint32_t dbad = (uint32_t)((a.D - 1 - (NCH - 1) * 64 - lane) >> 31);
 #pragma unroll
             for (int c = 0; c < NCH; c++) {
                 const int d = c * 64 + lane;
@@ -189,7 +196,9 @@ __global__ __launch_bounds__(64) void window_kernel(const WinArgs a)
 #pragma unroll
                 for (int t = 0; t < WIN_TX; t++) {
                     if (t > 0) s += col[c][t + NWIN - 1] - col[c][t - 1];
-                    if (d < a.D && d <= x0 + t) key[t] = min(key[t], (s << 9) | (uint32_t)d);
+                    uint32_t cand = (s << 9) | (uint32_t)d | (uint32_t)((rem + t - 64 * c) >> 31);
+                    if (c == NCH - 1) cand |= dbad;
+                    key[t] = min(key[t], cand);
                 }
             }
 #pragma unroll
