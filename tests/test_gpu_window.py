@@ -34,7 +34,8 @@ def test_ssd_volume(tv, dev, W, H, D, r):
     assert (got.cpu().numpy().view(np.uint32) == want).all()
 
 
-@pytest.mark.parametrize("W,H,D,r", [(61, 29, 32, 3), (140, 50, 200, 3), (17, 80, 64, 2)])
+@pytest.mark.parametrize("W,H,D,r", [(61, 29, 32, 3), (140, 50, 200, 3), (17, 80, 64, 2),
+                                     (300, 70, 256, 3), (45, 33, 100, 1)])
 def test_fast_wta_ssd(tv, dev, W, H, D, r):
     L, R = pair(W, H, D - 1, 7 * W + H)
     want = oracle.fast_wta_ssd(L, R, D, r)
@@ -121,3 +122,7 @@ def test_ssd_dp_window_1080p(tv, dev):
     got = tv.dp(Cg, 20).cpu().numpy()
     assert (got == want).all()
     assert (np.abs(got.astype(int) - gt) <= 1).mean() > 0.5
+    # fused SSD + WTA (no volume) at the full size: the lowest-index argmin of
+    # the same volume
+    fused = tv.fast_wta_ssd(Ld, Rd, 256, 3).cpu().numpy()
+    assert (fused == oracle.wta(C)).all()

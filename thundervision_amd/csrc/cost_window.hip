@@ -42,6 +42,34 @@ struct WinArgs {
     int zero_mean;
 };
 
+// one halving step of wave_min16 (lane partner l ^ B within 32 lanes)
+template <int N, int B>
+__device__ __forceinline__ void min_halve(uint32_t (&v)[2 * N], uint32_t (&o)[N], int lane)
+{
+    const bool up = lane & B; // keeps the upper half
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+        const uint32_t send = up ? v[j] : v[j + N];
+        const uint32_t keep = up ? v[j + N] : v[j];
+        const uint32_t got =
+            (uint32_t)__builtin_amdgcn_ds_swizzle((int)send, (B << 10) | 0x1F); // lane ^ B
+        o[j] = min(keep, got);
+    }
+}
+
+// Minimum over the wave of each of 16 per-lane keys; lane l returns the
+// minimum of key[(l >> 1) & 15].
+__device__ __forceinline__ uint32_t wave_min16(uint32_t (&key)[16], int lane)
+{
+    uint32_t k8[8], k4[4], k2[2], k1[1];
+    min_halve<8, 16>(key, k8, lane);
+    min_halve<4, 8>(k8, k4, lane);
+    min_halve<2, 4>(k4, k2, lane);
+    min_halve<1, 2>(k2, k1, lane);
+    uint32_t k = min(k1[0], (uint32_t)__builtin_amdgcn_ds_swizzle((int)k1[0], (1 << 10) | 0x1F));
+    return min(k, (uint32_t)__builtin_amdgcn_ds_bpermute((lane ^ 32) << 2, (int)k));
+}
+
 template <int RAD, int NCH, int MODE>
 __global__ __launch_bounds__(64) void window_kernel(const WinArgs a)
 {
@@ -201,18 +229,16 @@ __global__ __launch_bounds__(64) void window_kernel(const WinArgs a)
                     key[t] = min(key[t], cand);
                 }
             }
-#pragma unroll
-            for (int t = 0; t < WIN_TX; t++) {
-                // wave minimum (the 4 DPP row steps, then across the rows)
-                uint32_t k = row_min_u32(key[t]);
-                k = min(k, (uint32_t)__builtin_amdgcn_readlane((int)k, 0));
-                k = min(k, (uint32_t)__builtin_amdgcn_readlane((int)k, 16));
-                k = min(k, (uint32_t)__builtin_amdgcn_readlane((int)k, 32));
-                k = min(k, (uint32_t)__builtin_amdgcn_readlane((int)k, 48));
-                if (lane == t && x0 + t < W)
-                    ((int16_t *)a.out)[(size_t)y * a.disp_pitch + x0 + t] =
-                        (int16_t)(k == 0xFFFFFFFFu ? 0 : (k & 511u));
-            }
+            // the 16 wave minima at once: each halving step pairs lane l with
+            // l ^ b (b = 16, 8, 4, 2; ds_swizzle within 32 lanes), sends the
+            // half of its keys the partner keeps and keeps the other half, so
+            // lane l ends with column t = (l >> 1) & 15 of its 32-lane half;
+            // then l ^ 1 and l ^ 32
+            const uint32_t k = wave_min16(key, lane);
+            const int t = (lane >> 1) & 15;
+            if ((lane & 33) == 0 && x0 + t < W)
+                ((int16_t *)a.out)[(size_t)y * a.disp_pitch + x0 + t] =
+                    (int16_t)(k == 0xFFFFFFFFu ? 0 : (k & 511u));
         } else if constexpr (MODE == WIN_NCC) {
             // columns outer, chunks inner: a column's four right-value reads
             // go out together, and the left pixel's values (lane t holds
