@@ -332,19 +332,36 @@ __global__ __launch_bounds__(256) void box_sums_kernel(const uint8_t *__restrict
                                                        int H, int pitch, int zero_mean,
                                                        int32_t *__restrict__ box)
 {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    // separable: the clamped column sums of the block's 256 + 2r columns
+    // into LDS, then each pixel adds its 2r + 1 columns
+    constexpr int T = 256, SPAN = T + 2 * RAD;
+    __shared__ int4 cs[SPAN]; // {sum l, sum l^2, sum r, sum r^2} of one column
+    const int x0 = blockIdx.x * T, y = blockIdx.y;
+    for (int i = threadIdx.x; i < SPAN; i += T) {
+        const int xx = min(max(x0 - RAD + i, 0), W - 1);
+        int32_t cl = 0, cll = 0, cr = 0, crr = 0;
+#pragma unroll
+        for (int dy = -RAD; dy <= RAD; dy++) {
+            const size_t o = (size_t)min(max(y + dy, 0), H - 1) * pitch + xx;
+            const int32_t l = L[o], r = R[o];
+            cl += l;
+            cll += l * l;
+            cr += r;
+            crr += r * r;
+        }
+        cs[i] = make_int4(cl, cll, cr, crr);
+    }
+    __syncthreads();
+    const int x = x0 + threadIdx.x;
     if (x >= W) return;
     int32_t sl = 0, sll = 0, sr = 0, srr = 0;
-    for (int dy = -RAD; dy <= RAD; dy++) {
-        const int yy = min(max(y + dy, 0), H - 1);
-        for (int dx = -RAD; dx <= RAD; dx++) {
-            const int xx = min(max(x + dx, 0), W - 1);
-            const int32_t l = L[(size_t)yy * pitch + xx], r = R[(size_t)yy * pitch + xx];
-            sl += l;
-            sll += l * l;
-            sr += r;
-            srr += r * r;
-        }
+#pragma unroll
+    for (int i = 0; i <= 2 * RAD; i++) {
+        const int4 c = cs[threadIdx.x + i];
+        sl += c.x;
+        sll += c.y;
+        sr += c.z;
+        srr += c.w;
     }
     const int64_t N = (int64_t)(2 * RAD + 1) * (2 * RAD + 1);
     const int64_t vl = zero_mean ? N * sll - (int64_t)sl * sl : sll;
