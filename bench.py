@@ -178,9 +178,14 @@ def bench_window(tv, synth, torch, ctx, reps=5):
     dev = torch.device("cuda:0")
     Ld, Rd = torch.from_numpy(L).to(dev), torch.from_numpy(R).to(dev)
     cells = W * H * D
+    # SSD 7x7 costs (<= 49 * 255^2 < 2^22, invalid exactly where d > x) meet
+    # the DP's bounded-cost contract (32-bit sums); the general 64-bit form
+    # is timed beside it
     paths = {
-        "ssd_dp": (lambda: tv.dp(tv.cost_ssd(Ld, Rd, D, radius=3, ctx=ctx), p_occ=20, ctx=ctx),
-                   9),
+        "ssd_dp": (lambda: tv.dp(tv.cost_ssd(Ld, Rd, D, radius=3, ctx=ctx), p_occ=20,
+                                 flags=tv.DP_COST_BOUNDED, ctx=ctx), 9),
+        "ssd_dp_64bit_sums": (lambda: tv.dp(tv.cost_ssd(Ld, Rd, D, radius=3, ctx=ctx), p_occ=20,
+                                            ctx=ctx), 9),
         "ncc_wta": (lambda: tv.wta(tv.cost_ncc(Ld, Rd, D, radius=4, zero_mean=True, ctx=ctx),
                                    ctx=ctx), 8),
         "fused_ssd_wta": (lambda: tv.fast_wta_ssd(Ld, Rd, D, radius=3, ctx=ctx), None),

@@ -160,7 +160,13 @@ int tdvh_wta(tdvh_ctx *ctx, const tdvh_dsi *S, int mode, int16_t *disp,
 
 typedef enum {
     TDVH_DP_START_ZERO = 1,    /* dynamicprogcpu.cpp:67-79 */
-    TDVH_DP_REF_BACKTRACK = 2  /* dynamicprogcpu.cpp:81-91 */
+    TDVH_DP_REF_BACKTRACK = 2, /* dynamicprogcpu.cpp:81-91 */
+    /* The caller promises every cost is below 2^22 and INVALID (0xFFFFFFFF)
+     * exactly on the cells with d > x (true of tdvh_cost_ssd volumes with
+     * radius <= 3: at most 49 * 255^2); p_occ must be below 2^20. The sums
+     * then run in 32 bits with per-column renormalisation, with the same
+     * results. No counterpart in the reference (an execution choice). */
+    TDVH_DP_COST_BOUNDED = 4
 } tdvh_dp_flags;
 
 /* Scanline dynamic programming over uint32 costs (STRIDED) with occlusion

@@ -84,22 +84,51 @@ def test_ncc_window_1080p_crop_with_near_ties(tv, dev):
     assert ((d_gpu == d_ref) | near).all()
 
 
+@pytest.mark.parametrize("bounded", [0, 4])
 @pytest.mark.parametrize("W,H,D,flags", [(40, 12, 16, 0), (97, 20, 64, 0), (64, 9, 200, 0),
                                          (70, 15, 33, 1), (70, 15, 33, 2), (70, 15, 33, 3),
-                                         (1, 5, 8, 0), (130, 7, 256, 0)])
-def test_dp(tv, dev, W, H, D, flags):
+                                         (1, 5, 8, 0), (130, 7, 256, 0), (300, 6, 100, 0),
+                                         (700, 4, 256, 3)])
+def test_dp(tv, dev, W, H, D, flags, bounded):
+    """SSD volumes (costs <= 49 * 255^2, invalid exactly where d > x), in
+    64-bit sums and in the bounded-cost 32-bit form."""
     L, R = pair(W, H, D - 1, 11 * W + D)
     C = oracle.cost_ssd(L, R, D, 3)
     want = oracle.dp(C, 20, flags)
-    got = tv.dp(to_dev(C.view(np.int32), dev), 20, flags)
+    got = tv.dp(to_dev(C.view(np.int32), dev), 20, flags | bounded)
     assert (got.cpu().numpy() == want).all()
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_dp_bounded_extreme_costs(tv, dev, seed):
+    """The bounded-cost contract at its edges: costs up to 2^22 - 1 (far above
+    SSD's), long rows, p_occ up to 2^20 - 1, ties from a coarse cost alphabet,
+    against the oracle's 64-bit sums; both volume layouts."""
+    rng = np.random.default_rng(seed)
+    H, W, D = 5, 1500, 256
+    hi = [1 << 22, 4, 1 << 22][seed - 1]
+    p_occ = [20, 1, (1 << 20) - 1][seed - 1]
+    C = rng.integers(0, hi, (H, W, D), dtype=np.uint64).astype(np.uint32)
+    x = np.arange(W)[None, :, None]
+    d = np.arange(D)[None, None, :]
+    C[np.broadcast_to(d > x, C.shape)] = 0xFFFFFFFF
+    want = oracle.dp(C, p_occ)
+    got = tv.dp(to_dev(C.view(np.int32), dev), p_occ, tv.DP_COST_BOUNDED).cpu().numpy()
+    assert (got == want).all()
+    Cx = to_dev(np.ascontiguousarray(C.transpose(1, 0, 2)).view(np.int32), dev).permute(1, 0, 2)
+    assert (tv.dp(Cx, p_occ, tv.DP_COST_BOUNDED).cpu().numpy() == want).all()
+
+
+def test_dp_bounded_rejects_large_penalty(tv, dev):
+    C = to_dev(np.zeros((2, 8, 4), np.int32), dev)
+    with pytest.raises(RuntimeError, match="p_occ"):
+        tv.dp(C, 1 << 20, tv.DP_COST_BOUNDED)
 
 
 @pytest.mark.parametrize("hi,inv", [(1 << 20, 0.0), (1 << 31, 0.05), (0xFFFFFFF0, 0.3)])
 def test_dp_arbitrary_costs(tv, dev, hi, inv):
-    """Costs far above SSD's range (the 32-bit pass gives up on such rows and
-    the 64-bit pass redoes them) and scattered invalid cells, both layouts of
-    the volume's strides."""
+    """Costs far above SSD's range and scattered invalid cells (64-bit sums),
+    both layouts of the volume's strides."""
     rng = np.random.default_rng(hi % 1000 + int(inv * 100))
     H, W, D = 7, 300, 256
     C = rng.integers(0, hi, (H, W, D), dtype=np.uint64).astype(np.uint32)
@@ -121,6 +150,7 @@ def test_ssd_dp_window_1080p(tv, dev):
     want = oracle.dp(C, 20)
     got = tv.dp(Cg, 20).cpu().numpy()
     assert (got == want).all()
+    assert (tv.dp(Cg, 20, tv.DP_COST_BOUNDED).cpu().numpy() == want).all()
     assert (np.abs(got.astype(int) - gt) <= 1).mean() > 0.5
     # fused SSD + WTA (no volume) at the full size: the lowest-index argmin of
     # the same volume

@@ -9,6 +9,22 @@
 // then backtrack from the lowest-index argmin of A(W-1, .).
 // Accumulation is uint64, so no row can overflow (1920 * 49 * 255^2 > 2^32).
 //
+// Bounded costs (TDVH_DP_COST_BOUNDED: every cost below 2^22, INV32 exactly
+// on the cells with d > x, P < 2^20; 7x7 SSD costs are at most 3,186,225)
+// accumulate in uint32 instead. Subtracting one constant from a whole column
+// changes no comparison, so every 4 columns the wave minimum of 4 columns
+// earlier is subtracted (the column minimum never decreases, so nothing goes
+// below 0; the reduction is off the dependency chain). A valid A(x, d) then
+// exceeds the column minimum it is measured from by at most (D + 7)(Cmax + P):
+// d is reachable from that minimum's disparity within max(D - 1, 7) columns
+// through valid cells. So valid sums stay below 264 * (2^22 + 2^20) < 1.4e9,
+// while invalid ones stay above 2^32 - 1 - 8(Cmax + P) > 4.2e9: comparisons
+// between a valid and an invalid operand come out as with exact infinities,
+// and saturating adds keep INV32 + x and INF + P at INF. Invalid cells (d > x)
+// are recomputed to INF at every column from their INV32 cost, and a valid
+// cell's chosen predecessor is valid, so the backtrack never reads a
+// decision taken between two invalid operands.
+//
 // Mapping: one wave per row, lane l holds d = l*K + k (K <= 4) in registers;
 // the d-1 / d+1 neighbours across lanes move with whole-wave DPP shifts (the
 // vacated end lane reads the invalid value, which is exactly the missing
@@ -21,6 +37,8 @@
 // with the loads, take a quarter of its slots. The backtrack stages 64
 // columns of decisions in LDS at a time and walks them with a wave-uniform
 // disparity.
+#include <type_traits>
+
 #include "kernels.hpp"
 
 namespace tdvh {
@@ -55,9 +73,29 @@ __device__ __forceinline__ uint64_t wave_from_next(uint64_t v)
     return ((uint64_t)hi << 32) | lo;
 }
 
+__device__ __forceinline__ uint32_t wave_from_prev(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x138, 0xF, 0xF, false);
+}
+__device__ __forceinline__ uint32_t wave_from_next(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x130, 0xF, 0xF, false);
+}
+
+// a + b, or INF when either is INF (uint64) / saturating at INF (uint32)
+__device__ __forceinline__ uint64_t dp_add(uint64_t a, uint64_t b)
+{
+    return (a == DP_INF || b == DP_INF) ? DP_INF : a + b;
+}
+__device__ __forceinline__ uint32_t dp_add(uint32_t a, uint32_t b)
+{
+    return __builtin_elementwise_add_sat(a, b);
+}
+
 // VEC: K == 4, D == 256 and 16-byte aligned lanes (one 16-byte load per lane
-// and step); otherwise K word loads with a disparity bound.
-template <int K, int PF, bool VEC>
+// and step); otherwise K word loads with a disparity bound. B32: uint32
+// accumulation under the bounded-cost contract.
+template <int K, int PF, bool VEC, bool B32>
 __global__ __launch_bounds__(64) void dp_kernel(const uint32_t *__restrict__ C, int W, int H,
                                                 int D, int64_t sx, int64_t sy, uint32_t P,
                                                 int flags, uint8_t *__restrict__ path,
@@ -82,13 +120,16 @@ __global__ __launch_bounds__(64) void dp_kernel(const uint32_t *__restrict__ C, 
         }
     };
 
-    uint64_t A[K];
+    typedef typename std::conditional<B32, uint32_t, uint64_t>::type acc_t;
+    constexpr acc_t INF = (acc_t)DP_INF;
+    acc_t A[K];
     {
         uint32_t c[K];
         load(c, 0);
 #pragma unroll
-        for (int k = 0; k < K; k++) A[k] = c[k] == INV32 ? DP_INF : c[k];
+        for (int k = 0; k < K; k++) A[k] = c[k] == INV32 ? INF : c[k];
     }
+    uint32_t pend = 0; // B32: wave minimum of the last renormalised column
     uint32_t ring[PF][K];
 #pragma unroll
     for (int u = 0; u < PF; u++) load(ring[u], 1 + u);
@@ -101,18 +142,18 @@ __global__ __launch_bounds__(64) void dp_kernel(const uint32_t *__restrict__ C, 
 #pragma unroll
         for (int k = 0; k < K; k++) c[k] = ring[u][k];
         load(ring[u], x + PF);
-        const uint64_t left = wave_from_prev(A[K - 1]);
-        const uint64_t right = wave_from_next(A[0]);
+        const acc_t left = wave_from_prev(A[K - 1]);
+        const acc_t right = wave_from_next(A[0]);
         uint32_t codes = 0;
-        uint64_t An[K];
+        acc_t An[K];
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            const uint64_t lm = k == 0 ? left : A[k - 1];
-            const uint64_t lp = k == K - 1 ? right : A[k + 1];
-            const uint64_t c1 = lm != DP_INF ? lm + P : DP_INF;
-            const uint64_t c2 = A[k];
-            const uint64_t c3 = lp != DP_INF ? lp + P : DP_INF;
-            uint64_t mv;
+            const acc_t lm = k == 0 ? left : A[k - 1];
+            const acc_t lp = k == K - 1 ? right : A[k + 1];
+            const acc_t c1 = dp_add(lm, (acc_t)P);
+            const acc_t c2 = A[k];
+            const acc_t c3 = dp_add(lp, (acc_t)P);
+            acc_t mv;
             uint32_t code;
             if (c1 < c2 && c1 < c3) {
                 mv = c1;
@@ -124,11 +165,29 @@ __global__ __launch_bounds__(64) void dp_kernel(const uint32_t *__restrict__ C, 
                 mv = c3;
                 code = 2;
             }
-            An[k] = (c[k] == INV32 || mv == DP_INF) ? DP_INF : mv + c[k];
+            if constexpr (B32) An[k] = dp_add(mv, c[k]);
+            else An[k] = (c[k] == INV32 || mv == DP_INF) ? DP_INF : mv + c[k];
             codes |= code << (2 * k);
         }
 #pragma unroll
         for (int k = 0; k < K; k++) A[k] = An[k];
+        if constexpr (B32) {
+            if ((x & 3) == 3) {
+                // subtract the minimum taken 4 columns ago, then take this
+                // column's (consumed at the next word boundary)
+#pragma unroll
+                for (int k = 0; k < K; k++) A[k] -= pend;
+                uint32_t mn = A[0];
+#pragma unroll
+                for (int k = 1; k < K; k++) mn = min(mn, (uint32_t)A[k]);
+                mn = row_min_u32(mn);
+                const uint32_t m0 = (uint32_t)__builtin_amdgcn_readlane((int)mn, 0);
+                const uint32_t m1 = (uint32_t)__builtin_amdgcn_readlane((int)mn, 16);
+                const uint32_t m2 = (uint32_t)__builtin_amdgcn_readlane((int)mn, 32);
+                const uint32_t m3 = (uint32_t)__builtin_amdgcn_readlane((int)mn, 48);
+                pend = min(min(m0, m1), min(m2, m3));
+            }
+        }
         cw |= codes << (8 * (x & 3));
         if ((x & 3) == 3 || (tail && x == W - 1)) {
             prow[(size_t)(x >> 2) * 64 + l] = cw;
@@ -154,7 +213,7 @@ __global__ __launch_bounds__(64) void dp_kernel(const uint32_t *__restrict__ C, 
     // start: lowest-index argmin of the last column (or d = 0)
     int start = 0;
     if (!(flags & TDVH_DP_START_ZERO)) {
-        uint64_t bv = DP_INF;
+        acc_t bv = INF;
         int bd = 0x7FFFFFFF;
 #pragma unroll
         for (int k = 0; k < K; k++)
@@ -163,7 +222,9 @@ __global__ __launch_bounds__(64) void dp_kernel(const uint32_t *__restrict__ C, 
                 bd = l * K + k;
             }
         for (int off = 32; off > 0; off >>= 1) {
-            const uint64_t ov = shfl_u64(bv, l ^ off);
+            acc_t ov;
+            if constexpr (B32) ov = (uint32_t)__shfl((int)bv, l ^ off, 64);
+            else ov = shfl_u64(bv, l ^ off);
             const int od = __shfl_xor(bd, off, 64);
             if (ov < bv || (ov == bv && od < bd)) {
                 bv = ov;
@@ -200,6 +261,23 @@ __global__ __launch_bounds__(64) void dp_kernel(const uint32_t *__restrict__ C, 
     }
 }
 
+template <bool B32>
+static void launch_dp(const uint32_t *cp, int W, int H, int D, int64_t sx, int64_t sy,
+                      uint32_t P, int flags, uint8_t *path, int16_t *disp, int pitch, bool vec,
+                      hipStream_t s)
+{
+    constexpr int PF = 16;
+    switch ((D + 63) / 64) {
+    case 1: dp_kernel<1, PF, false, B32><<<H, 64, 0, s>>>(cp, W, H, D, sx, sy, P, flags, path, disp, pitch); break;
+    case 2: dp_kernel<2, PF, false, B32><<<H, 64, 0, s>>>(cp, W, H, D, sx, sy, P, flags, path, disp, pitch); break;
+    case 3: dp_kernel<3, PF, false, B32><<<H, 64, 0, s>>>(cp, W, H, D, sx, sy, P, flags, path, disp, pitch); break;
+    default:
+        if (vec) dp_kernel<4, PF, true, B32><<<H, 64, 0, s>>>(cp, W, H, D, sx, sy, P, flags, path, disp, pitch);
+        else dp_kernel<4, PF, false, B32><<<H, 64, 0, s>>>(cp, W, H, D, sx, sy, P, flags, path, disp, pitch);
+        break;
+    }
+}
+
 } // namespace tdvh
 
 using namespace tdvh;
@@ -215,34 +293,22 @@ extern "C" int tdvh_dp(tdvh_ctx *ctx, const tdvh_dsi *C, uint32_t p_occ, int fla
     const int W = C->width, H = C->height, D = C->depth;
     TDVH_REQUIRE(W > 0 && H > 0 && D > 0 && D <= 256, "bad volume %dx%dx%d (D <= 256)", W, H, D);
     TDVH_REQUIRE(disp_pitch >= W, "disp_pitch %d < width %d", disp_pitch, W);
-    TDVH_REQUIRE((flags & ~(TDVH_DP_START_ZERO | TDVH_DP_REF_BACKTRACK)) == 0, "bad flags 0x%x",
-                 flags);
+    TDVH_REQUIRE((flags & ~(TDVH_DP_START_ZERO | TDVH_DP_REF_BACKTRACK | TDVH_DP_COST_BOUNDED)) ==
+                     0,
+                 "bad flags 0x%x", flags);
+    const bool b32 = flags & TDVH_DP_COST_BOUNDED;
+    TDVH_REQUIRE(!b32 || p_occ < (1u << 20), "TDVH_DP_COST_BOUNDED needs p_occ < 2^20 (got %u)",
+                 p_occ);
     hipStream_t s = (hipStream_t)stream;
     void *pb = nullptr;
     TDVH_TRY(ctx->scratch("dp_path", dp_row_bytes(W) * H, &pb));
-    const int K = (D + 63) / 64;
     prof_begin(s);
     const uint32_t *cp = (const uint32_t *)C->ptr;
     const bool vec = D == 256 && C->stride_x % 4 == 0 && C->stride_y % 4 == 0 &&
                      ((uintptr_t)cp & 15) == 0;
-    constexpr int PF = 16;
-    switch (K) {
-#define CASE(k)                                                                                    \
-    case k:                                                                                        \
-        dp_kernel<k, PF, false><<<H, 64, 0, s>>>(cp, W, H, D, C->stride_x, C->stride_y, p_occ,     \
-                                                 flags, (uint8_t *)pb, disp, disp_pitch);          \
-        break;
-        CASE(1) CASE(2) CASE(3)
-#undef CASE
-    case 4:
-        if (vec)
-            dp_kernel<4, PF, true><<<H, 64, 0, s>>>(cp, W, H, D, C->stride_x, C->stride_y, p_occ,
-                                                    flags, (uint8_t *)pb, disp, disp_pitch);
-        else
-            dp_kernel<4, PF, false><<<H, 64, 0, s>>>(cp, W, H, D, C->stride_x, C->stride_y,
-                                                     p_occ, flags, (uint8_t *)pb, disp,
-                                                     disp_pitch);
-        break;
-    }
+    if (b32) launch_dp<true>(cp, W, H, D, C->stride_x, C->stride_y, p_occ, flags, (uint8_t *)pb,
+                             disp, disp_pitch, vec, s);
+    else launch_dp<false>(cp, W, H, D, C->stride_x, C->stride_y, p_occ, flags, (uint8_t *)pb,
+                          disp, disp_pitch, vec, s);
     return check_launch("dp_kernel", s);
 }

@@ -17,7 +17,7 @@ LAYOUT_STRIDED, LAYOUT_SPLIT16 = 0, 1
 SGM_REF_GEOMETRY, SGM_START_ASSIGN, SGM_MIN_SKIP_LAST = 1, 2, 4
 SGM_REF_ALL = SGM_REF_GEOMETRY | SGM_START_ASSIGN | SGM_MIN_SKIP_LAST
 WTA_LOWEST, WTA_TREE = 0, 1
-DP_START_ZERO, DP_REF_BACKTRACK = 1, 2
+DP_START_ZERO, DP_REF_BACKTRACK, DP_COST_BOUNDED = 1, 2, 4
 FUSE_STAGED, FUSE_COST = 0, 1
 MAX_DEPTH = 256
 
