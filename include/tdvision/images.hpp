@@ -63,6 +63,11 @@ public:
     void *toCpuMem() const;
     FloatImage leftOrigin() const;
     void leftOrigin(FloatImage img);
+    // Largest valid cost the producing stage writes, with INVALID exactly on
+    // the cells with d > x (0 = unknown): lets a consumer choose narrower
+    // exact arithmetic (DynamicProgDev's 32-bit sums).
+    uint32_t costBound() const;
+    void costBound(uint32_t bound);
 
     struct Impl;
 

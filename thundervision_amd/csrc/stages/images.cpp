@@ -195,6 +195,7 @@ struct DSIMem::Impl {
     int dtype = TDVH_F32, layout = TDVH_LAYOUT_STRIDED;
     DevBuf buf;
     FloatImage leftOrigin;
+    uint32_t costBound = 0;
 };
 
 DSIMem::DSIMem() {}
@@ -266,6 +267,11 @@ void *DSIMem::toCpuMem() const
 }
 
 FloatImage DSIMem::leftOrigin() const { return m_impl ? m_impl->leftOrigin : FloatImage(); }
+uint32_t DSIMem::costBound() const { return m_impl ? m_impl->costBound : 0; }
+void DSIMem::costBound(uint32_t bound)
+{
+    if (m_impl) m_impl->costBound = bound;
+}
 void DSIMem::leftOrigin(FloatImage img)
 {
     if (m_impl) m_impl->leftOrigin = img;

@@ -50,6 +50,9 @@ void SSDDev::updateImpl(FloatImage left, FloatImage right, DSIMem dsi)
     checkTdvh(tdvh_cost_ssd(deviceContext(), left.devMemU8(), right.devMemU8(), W, d.height, W,
                             m_radius, &d, nullptr),
               "tdvh_cost_ssd");
+    // (2r + 1)^2 * 255^2 at most; INVALID exactly where d > x
+    const uint64_t side = 2 * (uint64_t)m_radius + 1, bound = side * side * 255 * 255;
+    dsi.costBound(bound <= 0xFFFFFFFEull ? (uint32_t)bound : 0);
 }
 
 void CrossCorrelationDev::updateImpl(FloatImage left, FloatImage right, DSIMem dsi)
@@ -100,7 +103,11 @@ void WTADev::updateImpl(DSIMem dsi, FloatImage out)
 void DynamicProgDev::updateImpl(DSIMem dsi, FloatImage out)
 {
     tdvh_dsi c = dsi.mem();
-    checkTdvh(tdvh_dp(deviceContext(), &c, m_pocc, m_flags, out.disparityDev(), c.width, nullptr),
+    int flags = m_flags;
+    // costs below 2^22 from a known producer: 32-bit sums, same results
+    const uint32_t bound = dsi.costBound();
+    if (bound && bound < (1u << 22) && m_pocc < (1u << 20)) flags |= TDVH_DP_COST_BOUNDED;
+    checkTdvh(tdvh_dp(deviceContext(), &c, m_pocc, flags, out.disparityDev(), c.width, nullptr),
               "tdvh_dp");
 }
 
