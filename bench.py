@@ -189,6 +189,8 @@ def bench_window(tv, synth, torch, ctx, reps=5):
         "ncc_wta": (lambda: tv.wta(tv.cost_ncc(Ld, Rd, D, radius=4, zero_mean=True, ctx=ctx),
                                    ctx=ctx), 8),
         "fused_ssd_wta": (lambda: tv.fast_wta_ssd(Ld, Rd, D, radius=3, ctx=ctx), None),
+        "fused_ncc_wta": (lambda: tv.fast_wta_ncc(Ld, Rd, D, radius=4, zero_mean=True, ctx=ctx),
+                          None),
     }
     out = {"workload": "1920x1080 uint8 pair, D=256, seed 3"}
     for name, (fn, bpc) in paths.items():

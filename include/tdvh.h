@@ -185,6 +185,16 @@ int tdvh_fast_wta_ssd(tdvh_ctx *ctx, const uint8_t *left,
                       int depth, int radius, int16_t *disp, int disp_pitch,
                       void *stream);
 
+/* NCC window (zero-mean or plain, as tdvh_cost_ncc) + lowest-index WTA
+ * without a cost volume: the disparities of tdvh_cost_ncc followed by
+ * tdvh_wta (TDVH_WTA_LOWEST). No counterpart in the reference (it fuses only
+ * SSD, fastwta.cu); the CrossCorrelationDev + WTADev chain computes the same
+ * map through a volume (crosscorrelation.cu:53-76, wta.cu:90-100). */
+int tdvh_fast_wta_ncc(tdvh_ctx *ctx, const uint8_t *left,
+                      const uint8_t *right, int width, int height, int pitch,
+                      int depth, int radius, int zero_mean, int16_t *disp,
+                      int disp_pitch, void *stream);
+
 typedef enum {
     TDVH_FUSE_STAGED = 0, /* BT volume in HBM, then SGM, then WTA */
     TDVH_FUSE_COST = 1    /* BT recomputed inside the SGM scan; no C volume */

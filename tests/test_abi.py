@@ -23,7 +23,8 @@ def declared_functions():
 def test_header_declares_the_entry_points():
     names = declared_functions()
     for must in ("tdvh_cost_bt", "tdvh_sgm", "tdvh_wta", "tdvh_dp", "tdvh_cost_ssd",
-                 "tdvh_cost_ncc", "tdvh_fast_wta_ssd", "tdvh_stereo_bt_sgm", "tdvh_batch_run"):
+                 "tdvh_cost_ncc", "tdvh_fast_wta_ssd", "tdvh_fast_wta_ncc", "tdvh_stereo_bt_sgm",
+                 "tdvh_batch_run"):
         assert must in names
 
 

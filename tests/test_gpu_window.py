@@ -84,6 +84,51 @@ def test_ncc_window_1080p_crop_with_near_ties(tv, dev):
     assert ((d_gpu == d_ref) | near).all()
 
 
+@pytest.mark.parametrize("W,H,D,r,zm", [(45, 30, 16, 4, True), (80, 41, 64, 4, True),
+                                        (64, 40, 100, 1, False), (33, 33, 256, 4, True),
+                                        (50, 20, 32, 2, False), (130, 70, 200, 5, True),
+                                        (300, 40, 256, 4, True), (5, 9, 3, 3, True)])
+def test_fast_wta_ncc(tv, dev, W, H, D, r, zm):
+    """Fused NCC + lowest-index WTA: bit-exact to the volume path (cost_ncc,
+    then wta), including depths that pad the last chunk and the x < d border."""
+    L, R = pair(W, H, D - 1, 5 * W + r)
+    Ld, Rd = to_dev(L, dev), to_dev(R, dev)
+    want = tv.wta(tv.cost_ncc(Ld, Rd, D, r, zm)).cpu().numpy()
+    got = tv.fast_wta_ncc(Ld, Rd, D, r, zm).cpu().numpy()
+    assert (got == want).all()
+
+
+def test_fast_wta_ncc_ties_and_flat_regions(tv, dev):
+    """Exact ties go to the lowest d, and all-invalid pixels give d = 0, as
+    in the volume path's WTA. The ties come from a texture periodic along x:
+    identical windows give identical costs at d = 3, 11, 19, ... Zero-variance
+    windows in a flat block make every cost of a pixel invalid."""
+    rng = np.random.default_rng(9)
+    H, W, D = 40, 200, 64
+    tile = rng.integers(0, 256, (H, 8), dtype=np.uint8)
+    L = np.ascontiguousarray(np.tile(tile, (1, W // 8)))
+    R = np.ascontiguousarray(np.roll(L, -3, axis=1))
+    L[:, 150:] = 77
+    R[:, 140:] = 77
+    Ld, Rd = to_dev(L, dev), to_dev(R, dev)
+    for zm in (True, False):
+        want = tv.wta(tv.cost_ncc(Ld, Rd, D, 4, zm)).cpu().numpy()
+        got = tv.fast_wta_ncc(Ld, Rd, D, 4, zm).cpu().numpy()
+        assert (got == want).all()
+        assert (want[:, 20:140] % 8 == 3).mean() > 0.9  # the ties were there to break
+
+
+def test_fast_wta_ncc_1080p(tv, dev):
+    """1920x1080, D = 256, 9x9 zero-mean: fused equals the volume path."""
+    L, R, _ = synth.workload_pair(synth.WINDOW_1080P)
+    Ld, Rd = to_dev(L, dev), to_dev(R, dev)
+    C = tv.cost_ncc(Ld, Rd, 256, 4, True)
+    want = tv.wta(C).cpu().numpy()
+    del C
+    got = tv.fast_wta_ncc(Ld, Rd, 256, 4, True).cpu().numpy()
+    assert (got == want).all()
+
+
 @pytest.mark.parametrize("bounded", [0, 4])
 @pytest.mark.parametrize("W,H,D,flags", [(40, 12, 16, 0), (97, 20, 64, 0), (64, 9, 200, 0),
                                          (70, 15, 33, 1), (70, 15, 33, 2), (70, 15, 33, 3),

@@ -1,5 +1,7 @@
-// Window matching costs: SSD (ssd.cu:15-50), NCC (crosscorrelation.cu:11-49)
-// and the fused SSD + lowest-index WTA of fastwta.cu:32-53.
+// Window matching costs: SSD (ssd.cu:15-50), NCC (crosscorrelation.cu:11-49),
+// the fused SSD + lowest-index WTA of fastwta.cu:32-53, and the same fusion
+// for NCC (no counterpart in the reference; its disparities are those of the
+// NCC volume followed by the lowest-index WTA of wtacpu.cpp:20-33).
 //
 // Mapping: one wave per tile of WIN_TX consecutive x and WIN_TY output rows;
 // lane i holds disparity c*64 + i of every chunk c, so the 64 lanes write 64
@@ -28,7 +30,17 @@
 
 namespace tdvh {
 
-enum { WIN_SSD = 0, WIN_SSD_WTA = 1, WIN_NCC = 2 };
+enum { WIN_SSD = 0, WIN_SSD_WTA = 1, WIN_NCC = 2, WIN_NCC_WTA = 3 };
+
+// Key of a float cost in the lowest-index WTA's order: +inf (invalid) and
+// padding above every finite cost, finite costs ordered through their bits
+// (NCC costs are never -0 or NaN).
+__device__ __forceinline__ uint32_t ncc_key(float v)
+{
+    if (!(v < __builtin_inff())) return 0xFFFFFFFFu;
+    const uint32_t b = __builtin_bit_cast(uint32_t, v);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
 constexpr int WIN_TX = 16; // output columns per wave
 constexpr int WIN_TY = 32; // output rows per wave
 
@@ -70,8 +82,11 @@ __device__ __forceinline__ uint32_t wave_min16(uint32_t (&key)[16], int lane)
     return min(k, (uint32_t)__builtin_amdgcn_ds_bpermute((lane ^ 32) << 2, (int)k));
 }
 
+// Two waves per SIMD: the other modes fit in 256 registers as they are; the
+// fused NCC + WTA then spills ~160 bytes per lane, and runs 0.85 ms at
+// 1920x1080 D = 256 against 0.95 ms at one wave per SIMD without spills.
 template <int RAD, int NCH, int MODE>
-__global__ __launch_bounds__(64) void window_kernel(const WinArgs a)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void window_kernel(const WinArgs a)
 {
     constexpr int NWIN = 2 * RAD + 1;
     constexpr int NC = WIN_TX + 2 * RAD;    // columns x0-RAD .. x0+WIN_TX-1+RAD
@@ -82,7 +97,8 @@ __global__ __launch_bounds__(64) void window_kernel(const WinArgs a)
     __shared__ uint32_t ring[SLOTS][ROW];
     // NCC: the right image's box sums and reciprocal roots of the output row, for right
     // pixels x0 - (NCH*64 - 1) .. x0 + WIN_TX - 1
-    constexpr int BW = MODE == WIN_NCC ? WIN_TX + NCH * 64 - 1 : 1;
+    constexpr bool NCC = MODE == WIN_NCC || MODE == WIN_NCC_WTA;
+    constexpr int BW = NCC ? WIN_TX + NCH * 64 - 1 : 1;
     constexpr int B_PER_LANE = (BW + 63) / 64;
     __shared__ int32_t sbr[BW];
     __shared__ float sqr[BW];
@@ -122,7 +138,7 @@ __global__ __launch_bounds__(64) void window_kernel(const WinArgs a)
     // pixel term of column i, this lane's disparity in chunk c, ring slot
     // pixel term of a left and a right value (bytes, so 24-bit products)
     auto term = [](int lv, int rv) -> uint32_t {
-        if constexpr (MODE == WIN_NCC) return (uint32_t)__mul24(lv, rv);
+        if constexpr (NCC) return (uint32_t)__mul24(lv, rv);
         else return (uint32_t)__mul24(lv - rv, lv - rv);
     };
 
@@ -144,7 +160,7 @@ __global__ __launch_bounds__(64) void window_kernel(const WinArgs a)
         // after the term updates (their latency hides behind that work)
         int32_t bpre[B_PER_LANE], lpre = 0;
         uint32_t qpre[B_PER_LANE], lqpre = 0;
-        if constexpr (MODE == WIN_NCC) {
+        if constexpr (NCC) {
             if (v - RAD >= y0) {
                 const size_t pl = (size_t)W * H, row = (size_t)(v - RAD) * W;
                 if (lane < WIN_TX) {
@@ -193,7 +209,7 @@ __global__ __launch_bounds__(64) void window_kernel(const WinArgs a)
         }
         const int y = v - RAD; // output row whose window is complete
         if (y < y0) continue;
-        if constexpr (MODE == WIN_NCC) {
+        if constexpr (NCC) {
 #pragma unroll
             for (int q = 0; q < B_PER_LANE; q++) {
                 const int k = lane + 64 * q;
@@ -239,6 +255,64 @@ __global__ __launch_bounds__(64) void window_kernel(const WinArgs a)
             if ((lane & 33) == 0 && x0 + t < W)
                 ((int16_t *)a.out)[(size_t)y * a.disp_pitch + x0 + t] =
                     (int16_t)(k == 0xFFFFFFFFu ? 0 : (k & 511u));
+        } else if constexpr (MODE == WIN_NCC_WTA) {
+            // the NCC costs of the tile's 16 columns (columns past the image
+            // end read clamped values and are never written); per lane and
+            // column the lowest chunk among equal minima, then the wave's
+            // minimum key per column and the lowest d holding it
+            const int32_t mult = a.zero_mean ? NWIN * NWIN : 1;
+            uint32_t sum[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; c++) {
+                sum[c] = 0;
+#pragma unroll
+                for (int i = 0; i < NWIN; i++) sum[c] += col[c][i];
+            }
+            // per lane and column: the best cost (float order; +inf never
+            // improves on the initial +inf) and its chunk (2 bits per column
+            // in one word, to keep the wave within 256 registers)
+            float best[WIN_TX];
+            uint32_t bcs = 0;
+#pragma unroll
+            for (int t = 0; t < WIN_TX; t++) {
+                if (t > 0) {
+#pragma unroll
+                    for (int c = 0; c < NCH; c++) sum[c] += col[c][t + NWIN - 1] - col[c][t - 1];
+                }
+                const int32_t sl = a.zero_mean ? __builtin_amdgcn_readlane(lpre, t) : 0;
+                const float ql =
+                    __builtin_bit_cast(float, __builtin_amdgcn_readlane((int)lqpre, t));
+                best[t] = __builtin_inff();
+#pragma unroll
+                for (int c = 0; c < NCH; c++) {
+                    const int d = c * 64 + lane;
+                    const int k = t + NCH * 64 - 1 - d;
+                    const int32_t num = __mul24(mult, (int32_t)sum[c]) - __mul24(sl, sbr[k]);
+                    const float inv = __fmul_rn(ql, sqr[k]);
+                    const float cost = inv != __builtin_inff() ? __fmaf_rn(-(float)num, inv, 1.0f)
+                                                               : __builtin_inff();
+                    // only the last chunk can hold d >= D (never a minimum)
+                    const bool better = cost < best[t] && (c < NCH - 1 || d < a.D);
+                    best[t] = better ? cost : best[t];
+                    // chunk 0 leaves the column's bits at their initial 0
+                    if (c > 0)
+                        bcs = better ? (bcs & ~(3u << (2 * t))) | ((uint32_t)c << (2 * t)) : bcs;
+                }
+            }
+            uint32_t key[WIN_TX];
+#pragma unroll
+            for (int t = 0; t < WIN_TX; t++) key[t] = ncc_key(best[t]);
+            // lane 2t holds column t's minimum key after the first butterfly
+            const uint32_t kmin = wave_min16(key, lane);
+#pragma unroll
+            for (int t = 0; t < WIN_TX; t++) {
+                const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)kmin, 2 * t);
+                key[t] = key[t] == m ? ((bcs >> (2 * t)) & 3u) * 64 + (uint32_t)lane : 0xFFFFu;
+            }
+            const uint32_t dmin = wave_min16(key, lane);
+            const int t = (lane >> 1) & 15;
+            if ((lane & 33) == 0 && x0 + t < W)
+                ((int16_t *)a.out)[(size_t)y * a.disp_pitch + x0 + t] = (int16_t)dmin;
         } else if constexpr (MODE == WIN_NCC) {
             // columns outer, chunks inner: a column's four right-value reads
             // go out together, and the left pixel's values (lane t holds
@@ -386,9 +460,10 @@ static int launch_window_r(const WinArgs &a, hipStream_t s)
     case 4: window_kernel<RAD, 4, MODE><<<grid, 64, 0, s>>>(a); break;
     default: set_error("window costs support D <= 256 (got %d)", a.D); return TDVH_ERR_UNSUPPORTED;
     }
-    return check_launch(MODE == WIN_NCC ? "ncc_window_kernel"
-                                        : (MODE == WIN_SSD ? "ssd_window_kernel"
-                                                           : "ssd_wta_window_kernel"),
+    return check_launch(MODE == WIN_NCC       ? "ncc_window_kernel"
+                        : MODE == WIN_SSD     ? "ssd_window_kernel"
+                        : MODE == WIN_SSD_WTA ? "ssd_wta_window_kernel"
+                                              : "ncc_wta_window_kernel",
                         s);
 }
 
@@ -411,6 +486,29 @@ static int check_images(const uint8_t *l, const uint8_t *r, int W, int H, int pi
     TDVH_REQUIRE(W > 0 && H > 0 && pitch >= W, "bad image %dx%d pitch %d", W, H, pitch);
     TDVH_REQUIRE(D > 0 && D <= 256, "depth %d outside [1, 256]", D);
     return TDVH_OK;
+}
+
+// Per-pixel window sums and reciprocal roots of both images into the
+// context's scratch (box_sums_kernel).
+static int ncc_box_sums(tdvh_ctx *ctx, const uint8_t *left, const uint8_t *right, int width,
+                        int height, int pitch, int radius, int zero_mean, const int32_t **out,
+                        hipStream_t s)
+{
+    void *box = nullptr;
+    TDVH_TRY(ctx->scratch("ncc_box", (size_t)width * height * 16, &box));
+    dim3 grid((width + 255) / 256, height);
+    prof_begin(s);
+    switch (radius) {
+#define CASE(r)                                                                                    \
+    case r:                                                                                        \
+        box_sums_kernel<r><<<grid, 256, 0, s>>>(left, right, width, height, pitch, zero_mean,      \
+                                                 (int32_t *)box);                                  \
+        break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5)
+#undef CASE
+    }
+    *out = (const int32_t *)box;
+    return check_launch("box_sums_kernel", s);
 }
 
 } // namespace tdvh
@@ -457,21 +555,25 @@ extern "C" int tdvh_cost_ncc(tdvh_ctx *ctx, const uint8_t *left, const uint8_t *
                  "NCC output must be a strided float %dx%d volume", width, height);
     TDVH_REQUIRE(radius >= 1 && radius <= 5, "window radius %d outside [1, 5]", radius);
     hipStream_t s = (hipStream_t)stream;
-    void *box = nullptr;
-    TDVH_TRY(ctx->scratch("ncc_box", (size_t)width * height * 16, &box));
-    dim3 grid((width + 255) / 256, height);
-    prof_begin(s);
-    switch (radius) {
-#define CASE(r)                                                                                    \
-    case r:                                                                                        \
-        box_sums_kernel<r><<<grid, 256, 0, s>>>(left, right, width, height, pitch, zero_mean,      \
-                                                 (int32_t *)box);                                  \
-        break;
-        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5)
-#undef CASE
-    }
-    TDVH_TRY(check_launch("box_sums_kernel", s));
+    const int32_t *box = nullptr;
+    TDVH_TRY(ncc_box_sums(ctx, left, right, width, height, pitch, radius, zero_mean, &box, s));
     WinArgs a{left, right, width, height, pitch, out->depth, out->stride_x, out->stride_y,
-              out->ptr, 0, (const int32_t *)box, zero_mean};
+              out->ptr, 0, box, zero_mean};
     return launch_window<WIN_NCC>(a, radius, s);
+}
+
+extern "C" int tdvh_fast_wta_ncc(tdvh_ctx *ctx, const uint8_t *left, const uint8_t *right,
+                                 int width, int height, int pitch, int depth, int radius,
+                                 int zero_mean, int16_t *disp, int disp_pitch, void *stream)
+{
+    clear_error();
+    TDVH_TRY(use_device(ctx));
+    TDVH_TRY(check_images(left, right, width, height, pitch, depth));
+    TDVH_REQUIRE(disp && disp_pitch >= width, "bad disparity output");
+    TDVH_REQUIRE(radius >= 1 && radius <= 5, "window radius %d outside [1, 5]", radius);
+    hipStream_t s = (hipStream_t)stream;
+    const int32_t *box = nullptr;
+    TDVH_TRY(ncc_box_sums(ctx, left, right, width, height, pitch, radius, zero_mean, &box, s));
+    WinArgs a{left, right, width, height, pitch, depth, 0, 0, disp, disp_pitch, box, zero_mean};
+    return launch_window<WIN_NCC_WTA>(a, radius, s);
 }

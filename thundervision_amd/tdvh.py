@@ -76,6 +76,7 @@ SIGNATURES = {
     "tdvh_wta": (_I, [_VP, ct.POINTER(DSI), _I, _VP, _I, _VP]),
     "tdvh_dp": (_I, [_VP, ct.POINTER(DSI), _U32, _I, _VP, _I, _VP]),
     "tdvh_fast_wta_ssd": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, _I, _VP, _I, _VP]),
+    "tdvh_fast_wta_ncc": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _VP, _I, _VP]),
     "tdvh_stereo_bt_sgm": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, ct.POINTER(SgmParams), _I, _VP,
                                 _I, _VP]),
     "tdvh_disp_to_unit_float": (_I, [_VP, _VP, _I, _I, _I, _I, _VP, _I, _VP]),
@@ -318,6 +319,15 @@ def fast_wta_ssd(left, right, depth, radius=3, ctx=None, stream=None):
     disp = torch.empty((H, W), dtype=torch.int16, device=left.device)
     _check(lib().tdvh_fast_wta_ssd(_ctx(ctx), _ptr(left), _ptr(right), W, H, pitch, depth,
                                    radius, _ptr(disp), W, _stream(stream)))
+    return disp
+
+
+def fast_wta_ncc(left, right, depth, radius=4, zero_mean=True, ctx=None, stream=None):
+    """NCC + lowest-index WTA without a volume: wta(cost_ncc(...))."""
+    W, H, pitch = _img_args(left, right)
+    disp = torch.empty((H, W), dtype=torch.int16, device=left.device)
+    _check(lib().tdvh_fast_wta_ncc(_ctx(ctx), _ptr(left), _ptr(right), W, H, pitch, depth,
+                                   radius, int(zero_mean), _ptr(disp), W, _stream(stream)))
     return disp
 
 
