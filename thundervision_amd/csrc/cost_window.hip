@@ -24,13 +24,15 @@
 // (numerators exact in int32: |N*Slr|, Sl*Sr <= (121*255)^2 < 2^31), with
 // the per-pixel reciprocal square roots IEEE-rounded (root, then quotient)
 // once per pixel, their product IEEE-rounded, and cost = 1 - num * product in
-// one fused multiply-add; a zero variance (an infinite reciprocal root) or
-// x - d < 0 gives +inf.
+// one fused multiply-add; a zero variance or x - d < 0 gives +inf (their
+// reciprocal roots are staged as quiet NaN, which the product and the
+// multiply-add carry, and one IEEE minNum against +inf maps to +inf).
 #include "kernels.hpp"
 
 namespace tdvh {
 
 enum { WIN_SSD = 0, WIN_SSD_WTA = 1, WIN_NCC = 2, WIN_NCC_WTA = 3 };
+constexpr uint32_t NCC_NAN = 0x7FC00000u; // quiet NaN: reciprocal root of an invalid window
 
 // Key of a float cost in the lowest-index WTA's order: +inf (invalid) and
 // padding above every finite cost, finite costs ordered through their bits
@@ -50,7 +52,8 @@ struct WinArgs {
     int64_t sx, sy;     // output strides (elements)
     void *out;          // uint32 / float volume, or int16 disparities (WTA)
     int disp_pitch;
-    const int32_t *box; // NCC: 4 planes W*H: sum L, 1/sqrt(var L) (f32), sum R, 1/sqrt(var R) (f32)
+    const int32_t *box; // NCC: 4 planes W*H: sum L, 1/sqrt(var L) (f32), sum R, 1/sqrt(var R)
+                        // (f32); quiet NaN for a zero variance
     int zero_mean;
 };
 
@@ -174,9 +177,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void wi
                     const int u = x0 - (NCH * 64 - 1) + k; // only 0 <= u < W are read later
                     const size_t o = row + (u < 0 ? 0 : (u >= W ? W - 1 : u));
                     bpre[q] = a.box[2 * pl + o];
-                    // x - d < 0: an infinite reciprocal root makes the cost
+                    // x - d < 0: a NaN reciprocal root makes the cost
                     // invalid
-                    qpre[q] = u < 0 ? 0x7F800000u : (uint32_t)a.box[3 * pl + o];
+                    qpre[q] = u < 0 ? NCC_NAN : (uint32_t)a.box[3 * pl + o];
                 }
             }
         }
@@ -279,7 +282,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void wi
 #pragma unroll
                     for (int c = 0; c < NCH; c++) sum[c] += col[c][t + NWIN - 1] - col[c][t - 1];
                 }
-                const int32_t sl = a.zero_mean ? __builtin_amdgcn_readlane(lpre, t) : 0;
+                const int32_t nsl = a.zero_mean ? -__builtin_amdgcn_readlane(lpre, t) : 0;
                 const float ql =
                     __builtin_bit_cast(float, __builtin_amdgcn_readlane((int)lqpre, t));
                 best[t] = __builtin_inff();
@@ -287,10 +290,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void wi
                 for (int c = 0; c < NCH; c++) {
                     const int d = c * 64 + lane;
                     const int k = t + NCH * 64 - 1 - d;
-                    const int32_t num = __mul24(mult, (int32_t)sum[c]) - __mul24(sl, sbr[k]);
-                    const float inv = __fmul_rn(ql, sqr[k]);
-                    const float cost = inv != __builtin_inff() ? __fmaf_rn(-(float)num, inv, 1.0f)
-                                                               : __builtin_inff();
+                    const int32_t num = __mul24(nsl, sbr[k]) + __mul24(mult, (int32_t)sum[c]);
+                    // NaN for an invalid cell, which no comparison lets win
+                    const float cost = __fmaf_rn(-(float)num, __fmul_rn(ql, sqr[k]), 1.0f);
                     // only the last chunk can hold d >= D (never a minimum)
                     const bool better = cost < best[t] && (c < NCH - 1 || d < a.D);
                     best[t] = better ? cost : best[t];
@@ -320,6 +322,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void wi
             const int tn = min(WIN_TX, W - x0);
             // plain NCC: numerator = Slr (the left sum is taken as 0)
             const int32_t mult = a.zero_mean ? NWIN * NWIN : 1;
+            // +inf hidden from the compiler, which would otherwise fold
+            // minNum(x, +inf) to x (exact only when x is not NaN)
+            float pinf = __builtin_inff();
+            asm volatile("" : "+s"(pinf));
             uint32_t sum[NCH];
 #pragma unroll
             for (int c = 0; c < NCH; c++) {
@@ -336,7 +342,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void wi
                     for (int c = 0; c < NCH; c++) sum[c] += col[c][t + NWIN - 1] - col[c][t - 1];
                 }
                 if (t >= tn) break;
-                const int32_t sl = a.zero_mean ? __builtin_amdgcn_readlane(lpre, t) : 0;
+                const int32_t nsl = a.zero_mean ? -__builtin_amdgcn_readlane(lpre, t) : 0;
                 const float ql =
                     __builtin_bit_cast(float, __builtin_amdgcn_readlane((int)lqpre, t));
 #pragma unroll
@@ -348,11 +354,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void wi
                     const int32_t sr = sbr[k];
                     const float qr = sqr[k];
                     // 24-bit operands: sum < 121 * 65025 < 2^23, sl, sr < 2^15;
-                    // ql, qr: reciprocal roots (+inf for a zero variance)
-                    const int32_t num = __mul24(mult, (int32_t)sum[c]) - __mul24(sl, sr);
+                    // ql, qr: reciprocal roots (NaN for an invalid window)
+                    const int32_t num = __mul24(nsl, sr) + __mul24(mult, (int32_t)sum[c]);
                     const float inv = __fmul_rn(ql, qr);
-                    const float cost = inv != __builtin_inff() ? __fmaf_rn(-(float)num, inv, 1.0f)
-                                                               : __builtin_inff();
+                    const float cost = __builtin_fminf(__fmaf_rn(-(float)num, inv, 1.0f), pinf);
                     // only the last chunk can hold d >= D
                     if (c < NCH - 1 || d < a.D) *(float *)(p + t * px + c * 256) = cost;
                 }
@@ -399,7 +404,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void wi
 // roots of their (zero-mean or plain) second moments:
 //   box = {sum L, 1/sqrt(var L), sum R, 1/sqrt(var R)}, var = N*sum v^2 - (sum v)^2
 // or sum v^2, exact in int64, then the IEEE square root and the IEEE quotient
-// (a zero variance gives +inf).
+// (a zero variance gives a quiet NaN, which marks the window invalid).
 template <int RAD>
 __global__ __launch_bounds__(256) void box_sums_kernel(const uint8_t *__restrict__ L,
                                                        const uint8_t *__restrict__ R, int W,
@@ -442,9 +447,11 @@ __global__ __launch_bounds__(256) void box_sums_kernel(const uint8_t *__restrict
     const int64_t vr = zero_mean ? N * srr - (int64_t)sr * sr : srr;
     const size_t pl = (size_t)W * H, o = (size_t)y * W + x;
     box[o] = sl;
-    box[pl + o] = __builtin_bit_cast(int32_t, __fdiv_rn(1.0f, __fsqrt_rn((float)vl)));
+    box[pl + o] = vl == 0 ? (int32_t)NCC_NAN
+                          : __builtin_bit_cast(int32_t, __fdiv_rn(1.0f, __fsqrt_rn((float)vl)));
     box[2 * pl + o] = sr;
-    box[3 * pl + o] = __builtin_bit_cast(int32_t, __fdiv_rn(1.0f, __fsqrt_rn((float)vr)));
+    box[3 * pl + o] = vr == 0 ? (int32_t)NCC_NAN
+                              : __builtin_bit_cast(int32_t, __fdiv_rn(1.0f, __fsqrt_rn((float)vr)));
 }
 
 template <int RAD, int MODE>
