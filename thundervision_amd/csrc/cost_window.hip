@@ -359,7 +359,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void wi
                     const float inv = __fmul_rn(ql, qr);
                     const float cost = __builtin_fminf(__fmaf_rn(-(float)num, inv, 1.0f), pinf);
                     // only the last chunk can hold d >= D
-                    if (c < NCH - 1 || d < a.D) *(float *)(p + t * px + c * 256) = cost;
+                    if (c < NCH - 1 || d < a.D) __builtin_nontemporal_store(cost, (float *)(p + t * px + c * 256));
                 }
             }
         } else {
@@ -393,7 +393,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void wi
                     const uint32_t v = rem + t - 64 * c >= 0 ? sum[c] : INV32;
                     // only the last chunk can hold d >= D
                     if (c < NCH - 1 || c * 64 + lane < a.D)
-                        *(uint32_t *)(p + t * px + c * 256) = v;
+                        __builtin_nontemporal_store(v, (uint32_t *)(p + t * px + c * 256));
                 }
             }
         }
