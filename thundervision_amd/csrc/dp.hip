@@ -102,7 +102,6 @@ __global__ __launch_bounds__(64) void dp_kernel(const uint32_t *__restrict__ C, 
                                                 int16_t *__restrict__ disp, int pitch)
 {
     static_assert(!VEC || K == 4, "vector loads cover four cells");
-    __shared__ uint8_t sp[64][64];
     const int l = threadIdx.x;
     const int y = blockIdx.x;
     const uint32_t *crow = C + y * sy;
@@ -234,30 +233,40 @@ __global__ __launch_bounds__(64) void dp_kernel(const uint32_t *__restrict__ C, 
         start = bd == 0x7FFFFFFF ? 0 : bd;
     }
 
-    // backtrack over chunks of 64 columns staged in LDS
+    // backtrack over blocks of 64 columns (16 decision words per lane): each
+    // lane loads its block's words with independent loads, then the whole
+    // wave walks the columns in lockstep with a wave-uniform disparity z,
+    // taking column x's decision byte from lane z / K by readlane; lane j
+    // keeps the disparity of the block's column j and stores it once
     const bool refbt = flags & TDVH_DP_REF_BACKTRACK;
-    int z = start;
+    int z = __builtin_amdgcn_readfirstlane(start);
     int16_t *out = disp + (size_t)y * pitch;
-    for (int xc = W - 1; xc >= 0; xc -= 64) {
-        const int xlo = xc - 63 > 0 ? xc - 63 : 0;
-        __syncthreads();
-        for (int cx = xlo; cx <= xc; cx++)
-            sp[cx - xlo][l] = cx == 0 ? (uint8_t)0x55 // x = 0: "stay"
-                                      : (uint8_t)(prow[(size_t)(cx >> 2) * 64 + l] >> (8 * (cx & 3)));
-        __syncthreads();
-        if (l == 0) {
-            for (int x = xc; x >= xlo; x--) {
-                const int code = (sp[x - xlo][z / K] >> (2 * (z % K))) & 3;
+    const int nw = (W + 3) / 4; // decision words per lane
+    for (int wb = ((W - 1) >> 2) & ~15; wb >= 0; wb -= 16) {
+        uint32_t wv[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) wv[i] = wb + i < nw ? prow[(size_t)(wb + i) * 64 + l] : 0u;
+        int mine = 0;
+#pragma unroll
+        for (int i = 15; i >= 0; i--) {
+#pragma unroll
+            for (int b = 3; b >= 0; b--) {
+                const int x = (wb + i) * 4 + b;
+                if (x > W - 1) continue; // wave-uniform
+                const uint32_t word = (uint32_t)__builtin_amdgcn_readlane((int)wv[i], z / K);
+                // x = 0 has no stored decision: "stay"
+                const int code = x == 0 ? 1 : (int)((word >> (8 * b + 2 * (z % K))) & 3u);
                 const int nz = z + code - 1;
                 if (refbt) {
                     if (nz >= 0 && nz < D) z = nz;
-                    out[x] = (int16_t)z;
+                    mine = l == 4 * i + b ? z : mine;
                 } else {
-                    out[x] = (int16_t)z;
+                    mine = l == 4 * i + b ? z : mine;
                     if (x > 0 && nz >= 0 && nz < D) z = nz;
                 }
             }
         }
+        if (4 * wb + l < W) out[4 * wb + l] = (int16_t)mine;
     }
 }
 
