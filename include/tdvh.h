@@ -133,16 +133,24 @@ typedef enum {
 } tdvh_sgm_flags;
 
 typedef struct {
-    int32_t p1, p2;   /* constant penalties */
+    int32_t p1, p2;   /* constant penalties of integer cost volumes: [0, 1024]
+                         for uint16, [0, 2^30] for uint32 */
     int32_t dir_mask; /* bit i: direction i, order (1,0) (-1,0) (0,1) (0,-1)
                          (1,1) (-1,-1) (-1,1) (1,-1); 0xFF = all 8 */
     int32_t flags;    /* tdvh_sgm_flags; 0 = standard geometry */
+    float fp1, fp2;   /* penalties of float cost volumes (finite, >= 0); the
+                         reference's are 30/255 and 150/255 (semiglobal.cu:254-255) */
 } tdvh_sgm_params;
 
 /* Semi-global aggregation + lowest-index WTA. C is uint16 (STRIDED or
- * SPLIT16). `S_out` (optional, may be NULL) receives the aggregated volume:
- * uint16 in standard geometry, int32 with TDVH_SGM_REF_GEOMETRY. `disp`
- * receives int16 disparities (row pitch in elements).
+ * SPLIT16; BT costs), uint32 (STRIDED; SSD costs) or float (STRIDED; NCC
+ * costs). `S_out` (optional, may be NULL) receives the aggregated volume:
+ * uint16 in standard geometry, int32 with TDVH_SGM_REF_GEOMETRY (uint16
+ * input only), uint32 or float (strided) for uint32 and float input.
+ * uint32 arithmetic saturates at 0xFFFFFFFF (the invalid value); float
+ * arithmetic is IEEE with the directions summed in increasing index. A step
+ * whose predecessor vector is entirely invalid restarts the path (L = C).
+ * `disp` receives int16 disparities (row pitch in elements).
  * Replaces SemiGlobalDevRun (semiglobal.cu:474-521). */
 int tdvh_sgm(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params *params,
              tdvh_dsi *S_out, int16_t *disp, int disp_pitch, void *stream);
@@ -161,20 +169,32 @@ int tdvh_wta(tdvh_ctx *ctx, const tdvh_dsi *S, int mode, int16_t *disp,
 typedef enum {
     TDVH_DP_START_ZERO = 1,    /* dynamicprogcpu.cpp:67-79 */
     TDVH_DP_REF_BACKTRACK = 2, /* dynamicprogcpu.cpp:81-91 */
-    /* The caller promises every cost is below 2^22 and INVALID (0xFFFFFFFF)
-     * exactly on the cells with d > x (true of tdvh_cost_ssd volumes with
-     * radius <= 3: at most 49 * 255^2); p_occ must be below 2^20. The sums
-     * then run in 32 bits with per-column renormalisation, with the same
-     * results. No counterpart in the reference (an execution choice). */
+    /* uint32 volumes whose costs the caller expects below 2^22 and INVALID
+     * (0xFFFFFFFF) exactly on the cells with d > x (true of tdvh_cost_ssd
+     * volumes with radius <= 3: at most 49 * 255^2), with p_occ below 2^20:
+     * the sums run in 32 bits with per-column renormalisation. The kernel
+     * checks that contract on every cost and recomputes each row that breaks
+     * it with 64-bit sums, so the results are the same either way. No
+     * counterpart in the reference (an execution choice). */
     TDVH_DP_COST_BOUNDED = 4
 } tdvh_dp_flags;
 
-/* Scanline dynamic programming over uint32 costs (STRIDED) with occlusion
+/* Scanline dynamic programming over a STRIDED cost volume with occlusion
  * penalty `p_occ` on +-1 transitions and the tie rule of
- * dynamicprogcpu.cpp:42-56; 64-bit accumulation.
+ * dynamicprogcpu.cpp:42-56; D <= 512. uint16 (BT) and uint32 (SSD) costs
+ * accumulate exactly (64-bit sums, or 32-bit sums with per-column
+ * renormalisation for uint16 volumes and under TDVH_DP_COST_BOUNDED, with
+ * every row that breaks the bound recomputed in 64 bits); float (NCC) costs
+ * accumulate in float with the penalty (float)p_occ.
  * Replaces DynamicProgDevRun (dynamicprog.cu:106-122). */
 int tdvh_dp(tdvh_ctx *ctx, const tdvh_dsi *C, uint32_t p_occ, int flags,
             int16_t *disp, int disp_pitch, void *stream);
+
+/* tdvh_dp over a float volume with a float occlusion penalty (finite,
+ * >= 0; the reference's DP has none, dynamicprogcpu.cpp:29-59, i.e. 0).
+ * Replaces DynamicProgDevRun on CrossCorrelationDev volumes. */
+int tdvh_dp_f32(tdvh_ctx *ctx, const tdvh_dsi *C, float p_occ, int flags,
+                int16_t *disp, int disp_pitch, void *stream);
 
 /* ---- fused matchers --------------------------------------------------- */
 

@@ -200,28 +200,39 @@ protected:
     void run(FloatImage left, FloatImage right, FloatImage out);
 };
 
+// commonstereomatcherfactory.hpp:9-55: any matching cost with any optimiser
+// on the device (commonstereomatcherfactory.cpp:39-81). The reference's CPU
+// optimiser modes (DynamicProgOnCPU, SemiGlobalOnCPU, WTAOnCPU) map to the
+// device-backed aliases SemiGlobalCPU / WTACPU / DynamicProgCPU, and Global
+// to SemiGlobal as there. The CPU compute mode (OpenCV matchers,
+// commonstereomatcherfactory.cpp:83-115) is not part of this library.
 class CommonStereoMatcherFactory {
 public:
-    enum MatchingMode { CPU, Device };
-    enum MatchingCostType { SSD, CrossCorrelationNorm, BirchfieldTomasi };
-    enum OptimizationType { WTA, DynamicProg, SemiGlobal };
+    enum ComputeDevMode { Device, CPU };
+    enum MatchMode { SSD, CrossCorrelationNorm, BirchfieldTomasi };
+    enum OptMode { WTA, DynamicProg, DynamicProgOnCPU, Global, SemiGlobal, SemiGlobalOnCPU, WTAOnCPU };
+    // the names this library used before (same values)
+    typedef ComputeDevMode MatchingMode;
+    typedef MatchMode MatchingCostType;
+    typedef OptMode OptimizationType;
 
+    // the reference's defaults (commonstereomatcherfactory.cpp:16-22) except
+    // the compute mode: Device, the only one here
     CommonStereoMatcherFactory()
-        : m_mode(Device), m_cost(BirchfieldTomasi), m_opt(SemiGlobal), m_maxDisparity(128)
+        : m_compMode(Device), m_matchMode(SSD), m_optMode(WTA), m_maxDisparity(64)
     {
     }
-    void computeDev(MatchingMode m) { m_mode = m; }
-    void matchingCost(MatchingCostType t) { m_cost = t; }
-    void optimization(OptimizationType t) { m_opt = t; }
+    void computeDev(ComputeDevMode m) { m_compMode = m; }
+    void matchingCost(MatchMode t) { m_matchMode = t; }
+    void optimization(OptMode t) { m_optMode = t; }
     void maxDisparity(int d) { m_maxDisparity = d; }
-    // Throws tdv::Exception for the CPU mode (OpenCV matchers are not part of
-    // this library).
+    // Throws tdv::Exception for the CPU compute mode.
     StereoMatcher *createStereoMatcher();
 
 private:
-    MatchingMode m_mode;
-    MatchingCostType m_cost;
-    OptimizationType m_opt;
+    ComputeDevMode m_compMode;
+    MatchMode m_matchMode;
+    OptMode m_optMode;
     int m_maxDisparity;
 };
 

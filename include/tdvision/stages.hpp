@@ -35,7 +35,10 @@ public:
 
 protected:
     virtual void updateImpl(FloatImage left, FloatImage right, DSIMem dsi) = 0;
-    virtual int dsiDtype() const = 0;
+    // Element type and layout of the volume the stage writes: float strided
+    // [x][y][d] by default, the reference's DSIMem (dsimem.cpp:22-42), so a
+    // subclass that only overrides updateImpl keeps its contract.
+    virtual int dsiDtype() const { return TDVH_F32; }
     virtual int dsiLayout() const { return TDVH_LAYOUT_STRIDED; }
 
 private:
@@ -123,15 +126,23 @@ private:
     Benchmark m_mark;
 };
 
-// SGM + WTA (semiglobaldev.cpp:16-35).
+// SGM + WTA (semiglobaldev.cpp:16-35) over any cost stage's volume: integer
+// penalties p1/p2 for uint16 (BT) and uint32 (SSD) costs, float penalties
+// for float (NCC) costs, by default the reference's 30/255 and 150/255
+// (semiglobal.cu:254-255).
 class SemiGlobalDev : public AbstractOptimizer {
 public:
     SemiGlobalDev(int p1 = 8, int p2 = 32, int flags = 0)
     {
         workName("Semi-global device");
-        m_params = tdvh_sgm_params{p1, p2, 0xFF, flags};
+        m_params = tdvh_sgm_params{p1, p2, 0xFF, flags, 30.0f / 255.0f, 150.0f / 255.0f};
     }
     void params(const tdvh_sgm_params &p) { m_params = p; }
+    void floatPenalties(float p1, float p2)
+    {
+        m_params.fp1 = p1;
+        m_params.fp2 = p2;
+    }
 
 protected:
     void updateImpl(DSIMem dsi, FloatImage outimg);
@@ -153,14 +164,18 @@ private:
     int m_mode;
 };
 
-// Scanline DP with occlusion penalty (dynamicprogdev.cpp:11-28).
+// Scanline DP with occlusion penalty (dynamicprogdev.cpp:11-28) over any
+// cost stage's volume: occlusionPenalty for integer (BT, SSD) costs, a float
+// penalty for float (NCC) costs, 0 by default as the reference's DP, which
+// has none (dynamicprogcpu.cpp:29-59).
 class DynamicProgDev : public AbstractOptimizer {
 public:
     explicit DynamicProgDev(uint32_t occlusionPenalty = 20, int flags = 0)
-        : m_pocc(occlusionPenalty), m_flags(flags)
+        : m_pocc(occlusionPenalty), m_poccf(0.0f), m_flags(flags)
     {
         workName("DynamicProg");
     }
+    void floatPenalty(float p) { m_poccf = p; }
 
 protected:
     void updateImpl(DSIMem dsi, FloatImage outimg);
@@ -168,8 +183,19 @@ protected:
 
 private:
     uint32_t m_pocc;
+    float m_poccf;
     int m_flags;
 };
+
+// The reference's CPU optimisers (semiglobalcpu.cpp:173-230,
+// wtacpu.cpp:14-41, dynamicprogcpu.cpp:94-112) under their names, run on the
+// device: same inputs and outputs, the same arithmetic as the device stages
+// (lowest-index WTA ties as wtacpu.cpp:28; DynamicProgCPU computes the DP its
+// reference compiles out, dynamicprogcpu.cpp:96-112). Nothing here runs on
+// the host.
+typedef SemiGlobalDev SemiGlobalCPU;
+typedef WTADev WTACPU;
+typedef DynamicProgDev DynamicProgCPU;
 
 TDV_NAMESPACE_END
 

@@ -86,6 +86,24 @@ def sgm_std(C, P1, P2, dir_mask=0xFF):
     return S
 
 
+def sgm_u32(C, P1, P2, dir_mask=0xFF):
+    """SGM over uint32 costs (saturating at INV32), directions summed in order."""
+    C = np.ascontiguousarray(C, np.uint32)
+    H, W, D = C.shape
+    S = np.empty_like(C)
+    lib().tdvref_sgm_u32(_p(C), W, H, D, ct.c_uint32(P1), ct.c_uint32(P2), dir_mask, _p(S))
+    return S
+
+
+def sgm_f32(C, P1, P2, dir_mask=0xFF):
+    """SGM over float32 costs (IEEE single precision, fixed operation order)."""
+    C = np.ascontiguousarray(C, np.float32)
+    H, W, D = C.shape
+    S = np.empty_like(C)
+    lib().tdvref_sgm_f32(_p(C), W, H, D, ct.c_float(P1), ct.c_float(P2), dir_mask, _p(S))
+    return S
+
+
 def sgm_ref(C, P1, P2, flags=SGM_START_ASSIGN | SGM_MIN_SKIP_LAST):
     C = np.ascontiguousarray(C, np.uint16)
     H, W, D = C.shape
@@ -138,6 +156,21 @@ def dp(C, p_occ=20, flags=0, return_acc=False):
     lib().tdvref_dp(_p(C), W, H, D, ct.c_uint32(p_occ), flags, _p(disp),
                     _p(acc) if acc is not None else None)
     return (disp, acc) if return_acc else disp
+
+
+def dp_u16(C, p_occ=20, flags=0):
+    """DP over uint16 (BT) costs: INV16 read as INV32, then tdvref_dp."""
+    C = np.asarray(C, np.uint16)
+    C32 = np.where(C == INV16, np.uint32(INV32), C.astype(np.uint32))
+    return dp(C32, p_occ, flags)
+
+
+def dp_f32(C, p_occ=0.0, flags=0):
+    C = np.ascontiguousarray(C, np.float32)
+    H, W, D = C.shape
+    disp = np.empty((H, W), np.int16)
+    lib().tdvref_dp_f32(_p(C), W, H, D, ct.c_float(p_occ), flags, _p(disp))
+    return disp
 
 
 def fast_wta_ssd(L, R, D, radius=3):

@@ -254,8 +254,10 @@ void tdvref_sgm_lr(const uint16_t *C, int W, int H, int D, int P1, int P2,
             uint16_t *l = Lr + p * D;
             const int qx = x - dx, qy = y - dy;
             uint32_t mn = 0xFFFFFFFFu;
-            if (qx < 0 || qx >= W || qy < 0 || qy >= H) {
-                /* path start: L = C (semiglobalcpu.cpp:71-84) */
+            /* path start: L = C (semiglobalcpu.cpp:71-84); a predecessor
+             * with no valid cell restarts the path the same way */
+            if (qx < 0 || qx >= W || qy < 0 || qy >= H ||
+                mins[(size_t)qy * W + qx] == 0xFFFFFFFFu) {
                 for (int d = 0; d < D; d++) {
                     l[d] = c[d];
                     if (c[d] != TDVREF_INV16 && c[d] < mn) mn = c[d];
@@ -299,6 +301,129 @@ void tdvref_sgm_std(const uint16_t *C, int W, int H, int D, int P1, int P2,
             if (S[i] != TDVREF_INV16) S[i] = (uint16_t)(S[i] + Lr[i]);
     }
     free(Lr);
+}
+
+/* ------------------------------------------------------------------------ */
+/* SGM over uint32 and float volumes (semiglobalcpu.cpp:87-125 recurrence,   */
+/* any DSIMem as semiglobaldev.cpp:16-28 accepts)                            */
+/* ------------------------------------------------------------------------ */
+
+static inline uint32_t sat_add32(uint32_t a, uint32_t b)
+{
+    const uint64_t s = (uint64_t)a + b;
+    return s > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)s;
+}
+static inline uint32_t umin32(uint32_t a, uint32_t b) { return a < b ? a : b; }
+
+/* One direction; visit order as tdvref_sgm_lr. For each pixel: L = C at a
+ * path start or after a predecessor with no valid cell; otherwise
+ *   t = min(L'(d), min(L'(d-1), L'(d+1)) + P1), E = min(t - m, P2),
+ *   L = C + E
+ * with saturating additions (0xFFFFFFFF is invalid and absorbs). */
+static void sgm_u32_dir(const uint32_t *C, int W, int H, int D, uint32_t P1, uint32_t P2,
+                        int dir, uint32_t *Lr, uint32_t *mins)
+{
+    const int dx = tdvref_dir_dx[dir], dy = tdvref_dir_dy[dir];
+    const int y0 = dy >= 0 ? 0 : H - 1, ystep = dy >= 0 ? 1 : -1;
+    const int x0 = dx >= 0 ? 0 : W - 1, xstep = dx >= 0 ? 1 : -1;
+    for (int yi = 0, y = y0; yi < H; yi++, y += ystep)
+        for (int xi = 0, x = x0; xi < W; xi++, x += xstep) {
+            const size_t p = (size_t)y * W + x;
+            const uint32_t *c = C + p * D;
+            uint32_t *l = Lr + p * D;
+            const int qx = x - dx, qy = y - dy;
+            const int start = qx < 0 || qx >= W || qy < 0 || qy >= H ||
+                              mins[(size_t)qy * W + qx] == TDVREF_INV32;
+            uint32_t mn = TDVREF_INV32;
+            if (start) {
+                for (int d = 0; d < D; d++) l[d] = c[d];
+            } else {
+                const size_t q = (size_t)qy * W + qx;
+                const uint32_t *lp = Lr + q * D;
+                const uint32_t m = mins[q];
+                for (int d = 0; d < D; d++) {
+                    const uint32_t nm = d > 0 ? lp[d - 1] : TDVREF_INV32;
+                    const uint32_t np = d + 1 < D ? lp[d + 1] : TDVREF_INV32;
+                    const uint32_t t = umin32(lp[d], sat_add32(umin32(nm, np), P1));
+                    l[d] = sat_add32(c[d], umin32(t - m, P2));
+                }
+            }
+            for (int d = 0; d < D; d++) mn = umin32(mn, l[d]);
+            mins[p] = mn;
+        }
+}
+
+void tdvref_sgm_u32(const uint32_t *C, int W, int H, int D, uint32_t P1, uint32_t P2,
+                    int dir_mask, uint32_t *S)
+{
+    const size_t n = (size_t)W * H * D;
+    uint32_t *Lr = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    uint32_t *mins = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)W * H);
+    int first = 1;
+    for (int dir = 0; dir < 8; dir++) {
+        if (!(dir_mask & (1 << dir))) continue;
+        sgm_u32_dir(C, W, H, D, P1, P2, dir, Lr, mins);
+        for (size_t i = 0; i < n; i++) S[i] = first ? Lr[i] : sat_add32(S[i], Lr[i]);
+        first = 0;
+    }
+    free(Lr);
+    free(mins);
+}
+
+/* A float cost as an operand: finite values as they are, +inf otherwise. */
+static inline float fcost(float v) { return (v < INFINITY && v > -INFINITY) ? v : INFINITY; }
+static inline float fmin2(float a, float b) { return a < b ? a : b; }
+
+/* The float recurrence of sgm_u32_dir with IEEE additions (+inf invalid). */
+static void sgm_f32_dir(const float *C, int W, int H, int D, float P1, float P2, int dir,
+                        float *Lr, float *mins)
+{
+    const int dx = tdvref_dir_dx[dir], dy = tdvref_dir_dy[dir];
+    const int y0 = dy >= 0 ? 0 : H - 1, ystep = dy >= 0 ? 1 : -1;
+    const int x0 = dx >= 0 ? 0 : W - 1, xstep = dx >= 0 ? 1 : -1;
+    for (int yi = 0, y = y0; yi < H; yi++, y += ystep)
+        for (int xi = 0, x = x0; xi < W; xi++, x += xstep) {
+            const size_t p = (size_t)y * W + x;
+            const float *c = C + p * D;
+            float *l = Lr + p * D;
+            const int qx = x - dx, qy = y - dy;
+            const int start = qx < 0 || qx >= W || qy < 0 || qy >= H ||
+                              !(mins[(size_t)qy * W + qx] < INFINITY);
+            float mn = INFINITY;
+            if (start) {
+                for (int d = 0; d < D; d++) l[d] = fcost(c[d]);
+            } else {
+                const size_t q = (size_t)qy * W + qx;
+                const float *lp = Lr + q * D;
+                const float m = mins[q];
+                for (int d = 0; d < D; d++) {
+                    const float nm = d > 0 ? lp[d - 1] : INFINITY;
+                    const float np = d + 1 < D ? lp[d + 1] : INFINITY;
+                    const float t = fmin2(lp[d], fmin2(nm, np) + P1);
+                    const float e = fmin2(t - m, P2);
+                    l[d] = fcost(c[d]) + e;
+                }
+            }
+            for (int d = 0; d < D; d++) mn = fmin2(mn, l[d]);
+            mins[p] = mn;
+        }
+}
+
+void tdvref_sgm_f32(const float *C, int W, int H, int D, float P1, float P2, int dir_mask,
+                    float *S)
+{
+    const size_t n = (size_t)W * H * D;
+    float *Lr = (float *)malloc(sizeof(float) * n);
+    float *mins = (float *)malloc(sizeof(float) * (size_t)W * H);
+    int first = 1;
+    for (int dir = 0; dir < 8; dir++) {
+        if (!(dir_mask & (1 << dir))) continue;
+        sgm_f32_dir(C, W, H, D, P1, P2, dir, Lr, mins);
+        for (size_t i = 0; i < n; i++) S[i] = first ? Lr[i] : S[i] + Lr[i];
+        first = 0;
+    }
+    free(Lr);
+    free(mins);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -577,6 +702,73 @@ void tdvref_dp(const uint32_t *C, int W, int H, int D, uint32_t p_occ,
         int z = 0;
         if (!(flags & TDVREF_DP_START_ZERO)) {
             uint64_t bv = DP_INF;
+            for (int d = 0; d < D; d++)
+                if (prev[d] < bv) {
+                    bv = prev[d];
+                    z = d;
+                }
+        }
+        int16_t *out = disp + (size_t)y * W;
+        if (flags & TDVREF_DP_REF_BACKTRACK) {
+            for (int x = W - 1; x >= 0; x--) {
+                const int nz = z + path[(size_t)x * D + z];
+                if (nz >= 0 && nz < D) z = nz;
+                out[x] = (int16_t)z;
+            }
+        } else {
+            for (int x = W - 1; x >= 0; x--) {
+                out[x] = (int16_t)z;
+                const int nz = z + path[(size_t)x * D + z];
+                if (x > 0 && nz >= 0 && nz < D) z = nz;
+            }
+        }
+    }
+    free(prev);
+    free(cur);
+    free(path);
+}
+
+/* Float DP (the recurrence of tdvref_dp on float costs): +inf invalid,
+ * c1 = A(d-1) + P, c3 = A(d+1) + P and A = C + chosen each one IEEE addition,
+ * the same tie rule, start and backtrack. */
+void tdvref_dp_f32(const float *C, int W, int H, int D, float p_occ, int flags, int16_t *disp)
+{
+    float *prev = (float *)malloc(sizeof(float) * (size_t)D);
+    float *cur = (float *)malloc(sizeof(float) * (size_t)D);
+    int8_t *path = (int8_t *)malloc((size_t)W * D);
+    for (int y = 0; y < H; y++) {
+        const float *crow = C + (size_t)y * W * D;
+        for (int d = 0; d < D; d++) {
+            prev[d] = fcost(crow[d]);
+            path[d] = 0;
+        }
+        for (int x = 1; x < W; x++) {
+            for (int d = 0; d < D; d++) {
+                const float c1 = d > 0 ? prev[d - 1] + p_occ : INFINITY;
+                const float c2 = prev[d];
+                const float c3 = d < D - 1 ? prev[d + 1] + p_occ : INFINITY;
+                float m;
+                int8_t p;
+                if (c1 < c2 && c1 < c3) {
+                    m = c1;
+                    p = -1;
+                } else if (c2 < c3) {
+                    m = c2;
+                    p = 0;
+                } else {
+                    m = c3;
+                    p = 1;
+                }
+                cur[d] = fcost(crow[(size_t)x * D + d]) + m;
+                path[(size_t)x * D + d] = p;
+            }
+            float *t = prev;
+            prev = cur;
+            cur = t;
+        }
+        int z = 0;
+        if (!(flags & TDVREF_DP_START_ZERO)) {
+            float bv = INFINITY;
             for (int d = 0; d < D; d++)
                 if (prev[d] < bv) {
                     bv = prev[d];

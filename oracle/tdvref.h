@@ -79,6 +79,20 @@ void tdvref_sgm_lr(const uint16_t *C, int W, int H, int D, int P1, int P2,
 void tdvref_sgm_std(const uint16_t *C, int W, int H, int D, int P1, int P2,
                     int dir_mask, uint16_t *S);
 
+/* SGM over uint32 (SSD) costs: the recurrence of tdvref_sgm_lr evaluated as
+ *   t = min(L'(d), min(L'(d-1), L'(d+1)) + P1), E = min(t - m, P2), L = C + E
+ * with every addition saturating at TDVREF_INV32 (the invalid value), L = C
+ * at a path start and after a predecessor with no valid cell, and
+ * S = L_first + L_second + ... over the directions of dir_mask in
+ * increasing index. */
+void tdvref_sgm_u32(const uint32_t *C, int W, int H, int D, uint32_t P1, uint32_t P2,
+                    int dir_mask, uint32_t *S);
+
+/* The same over float (NCC) costs in IEEE single precision with that
+ * operation order; +inf (and NaN, -inf costs) invalid. */
+void tdvref_sgm_f32(const float *C, int W, int H, int D, float P1, float P2, int dir_mask,
+                    float *S);
+
 /* Reference-geometry flags. */
 #define TDVREF_SGM_START_ASSIGN 1  /* semiglobalcpu.cpp:81 */
 #define TDVREF_SGM_MIN_SKIP_LAST 2 /* semiglobalcpu.cpp:95-97 */
@@ -131,6 +145,11 @@ void tdvref_wta_tree_u16(const uint16_t *S, int W, int H, int D, int16_t *disp);
  * [y][d]. */
 void tdvref_dp(const uint32_t *C, int W, int H, int D, uint32_t p_occ,
                int flags, int16_t *disp, uint64_t *acc_out);
+
+/* Per-row DP over float costs in IEEE single precision (+inf invalid):
+ * c1 = A(x-1, d-1) + p_occ, c3 = A(x-1, d+1) + p_occ, A = C + chosen, with
+ * the tie rule, start and backtrack of tdvref_dp. */
+void tdvref_dp_f32(const float *C, int W, int H, int D, float p_occ, int flags, int16_t *disp);
 
 /* ---- fused SSD + WTA (fastwta.cu:32-53) ------------------------------------ */
 void tdvref_fast_wta_ssd(const uint8_t *L, const uint8_t *R, int W, int H,

@@ -14,6 +14,11 @@
 #include <hip/hip_runtime.h>
 
 #include "tdvision/matchers.hpp"
+// the reference's own header names (as its tests include them)
+#include "tdvision/dynamicprogcpu.hpp"
+#include "tdvision/semiglobalcpu.hpp"
+#include "tdvision/ssddev.hpp"
+#include "tdvision/wtacpu.hpp"
 
 using namespace tdv;
 
@@ -287,9 +292,7 @@ static void test_gpu(const std::string &dir)
     }
     { // factory
         CommonStereoMatcherFactory fac;
-        fac.maxDisparity(c.D);
-        fac.matchingCost(CommonStereoMatcherFactory::SSD);
-        fac.optimization(CommonStereoMatcherFactory::WTA);
+        fac.maxDisparity(c.D); // SSD + WTA are the reference's defaults
         std::unique_ptr<StereoMatcher> m(fac.createStereoMatcher());
         CHECK(same(run_matcher(*m, c, nullptr), slurp(dir + "/ssd_wta_pf.i16"),
                    "factory SSD+WTA"));
@@ -301,6 +304,70 @@ static void test_gpu(const std::string &dir)
             threw = true;
         }
         CHECK(threw);
+    }
+    { // every cost x optimiser pairing of the factory (commonstereomatcherfactory.cpp:39-81)
+        const char *costs[3] = {"ssd", "ncc", "bt"};
+        const char *opts[3] = {"wta", "dp", "sgm"};
+        const CommonStereoMatcherFactory::MatchMode mm[3] = {
+            CommonStereoMatcherFactory::SSD, CommonStereoMatcherFactory::CrossCorrelationNorm,
+            CommonStereoMatcherFactory::BirchfieldTomasi};
+        const CommonStereoMatcherFactory::OptMode om[3][2] = {
+            {CommonStereoMatcherFactory::WTA, CommonStereoMatcherFactory::WTAOnCPU},
+            {CommonStereoMatcherFactory::DynamicProg, CommonStereoMatcherFactory::DynamicProgOnCPU},
+            {CommonStereoMatcherFactory::SemiGlobal, CommonStereoMatcherFactory::SemiGlobalOnCPU}};
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                for (int v = 0; v < 2; v++) {
+                    CommonStereoMatcherFactory fac;
+                    fac.maxDisparity(c.D);
+                    fac.matchingCost(mm[i]);
+                    fac.optimization(om[j][v]);
+                    std::unique_ptr<StereoMatcher> m(fac.createStereoMatcher());
+                    const std::string what = std::string("factory ") + costs[i] + "+" + opts[j];
+                    CHECK(same(run_matcher(*m, c, nullptr),
+                               slurp(dir + "/pair_" + costs[i] + "_" + opts[j] + ".i16"),
+                               what.c_str()));
+                }
+    }
+    { // TestSemiGlobalDev.WithSSD and TestSemiGlobalCPU.WithSSD
+      // (testsemiglobal.cpp:20-31, testsemiglobalcpu.cpp:8-23): SSD into
+      // SGM, with the prefilter, at 64 and 256 disparities
+        for (int D : {64, 256}) {
+            DevStereoMatcher m;
+            m.setMatchingCost(std::make_shared<SSDDev>(D));
+            if (D == 64) m.setOptimizer(std::make_shared<SemiGlobalDev>());
+            else m.setOptimizer(std::make_shared<SemiGlobalCPU>());
+            Case cd = c;
+            cd.D = D;
+            CHECK(same(run_matcher(m, cd, nullptr),
+                       slurp(dir + "/ssd_sgm_pf_" + std::to_string(D) + ".i16"),
+                       D == 64 ? "SSDDev(64)+SemiGlobalDev" : "SSDDev(256)+SemiGlobalCPU"));
+        }
+        DevStereoMatcher w;
+        w.setMatchingCost(std::make_shared<SSDDev>(c.D));
+        w.setOptimizer(std::make_shared<WTACPU>());
+        CHECK(same(run_matcher(w, c, nullptr), slurp(dir + "/ssd_wta_pf.i16"), "SSD+WTACPU"));
+        DevStereoMatcher d;
+        d.setMatchingCost(std::make_shared<SSDDev>(c.D));
+        d.setOptimizer(std::make_shared<DynamicProgCPU>(20));
+        CHECK(same(run_matcher(d, c, nullptr), slurp(dir + "/ssd_dp_pf.i16"), "SSD+DynamicProgCPU"));
+    }
+    { // a cost stage that overrides only updateImpl keeps the float DSI default
+        struct ZeroCost : AbstractMatchingCost {
+            ZeroCost() : AbstractMatchingCost(4) {}
+            void updateImpl(FloatImage, FloatImage, DSIMem dsi)
+            {
+                checkHip(hipMemset(dsi.mem().ptr, 0, dsi.dim().size() * 4), "clear");
+            }
+        } zc;
+        ReadWritePipe<FloatImage> lp, rp;
+        zc.inputs(&lp, &rp);
+        lp.write(FloatImage::FromGray8((const uint8_t *)c.L.data(), c.W, c.H));
+        rp.write(FloatImage::FromGray8((const uint8_t *)c.R.data(), c.W, c.H));
+        CHECK(zc.update());
+        DSIMem dsi;
+        CHECK(zc.output()->read(&dsi));
+        CHECK(dsi.mem().dtype == TDVH_F32 && dsi.mem().layout == TDVH_LAYOUT_STRIDED);
     }
     { // DSIMem download in the reference layout [x][y][d]
         ReadWritePipe<FloatImage> lp, rp;

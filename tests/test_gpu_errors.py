@@ -82,7 +82,7 @@ def test_cost_bt_bad_output(tv, dev, imgs):
     still_works(tv, imgs)
 
 
-@pytest.mark.parametrize("depth,radius,pattern", [(0, 3, "depth"), (257, 3, "depth"),
+@pytest.mark.parametrize("depth,radius,pattern", [(0, 3, "depth"), (513, 3, "depth"),
                                                   (16, 0, "radius"), (16, 6, "radius")])
 def test_window_costs_bad_depth_radius(tv, dev, imgs, depth, radius, pattern):
     L, R = imgs
@@ -125,12 +125,48 @@ def test_dp_bad_arguments(tv, dev, imgs):
     expect_error(tv, tv.lib().tdvh_dp(ctx(tv), ct.byref(d), 20, 8, dp_, W, stream()), "flags")
     expect_error(tv, tv.lib().tdvh_dp(ctx(tv), ct.byref(d), 20, 0, dp_, W - 1, stream()),
                  "disp_pitch")
-    d16 = dsi(tv, C, W, H, 8, tv.U16)
-    expect_error(tv, tv.lib().tdvh_dp(ctx(tv), ct.byref(d16), 20, 0, dp_, W, stream()),
-                 "uint32")
-    big = torch.zeros((1, 4, 257), dtype=torch.int32, device=dev)
-    db = dsi(tv, big, 4, 1, 257, tv.U32)
-    expect_error(tv, tv.lib().tdvh_dp(ctx(tv), ct.byref(db), 20, 0, dp_, 4, stream()), "D <= 256")
+    di = dsi(tv, C, W, H, 8, tv.I32)
+    expect_error(tv, tv.lib().tdvh_dp(ctx(tv), ct.byref(di), 20, 0, dp_, W, stream()),
+                 "uint16, uint32 or float")
+    big = torch.zeros((1, 4, 513), dtype=torch.int32, device=dev)
+    db = dsi(tv, big, 4, 1, 513, tv.U32)
+    expect_error(tv, tv.lib().tdvh_dp(ctx(tv), ct.byref(db), 20, 0, dp_, 4, stream()), "D <= 512")
+    expect_error(tv, tv.lib().tdvh_dp_f32(ctx(tv), ct.byref(d), 0.5, 0, dp_, W, stream()),
+                 "float cost volume")
+    df = dsi(tv, C, W, H, 8, tv.F32)
+    expect_error(tv, tv.lib().tdvh_dp_f32(ctx(tv), ct.byref(df), float("inf"), 0, dp_, W,
+                                          stream()), "penalty")
+    still_works(tv, imgs)
+
+
+def test_sgm_wide_bad_arguments(tv, dev, imgs):
+    C = torch.zeros((H, W, 8), dtype=torch.int32, device=dev)
+    disp = torch.empty((H, W), dtype=torch.int16, device=dev)
+    dp_ = ct.c_void_p(disp.data_ptr())
+    d32 = dsi(tv, C, W, H, 8, tv.U32)
+    p = tv.sgm_params(flags=tv.SGM_REF_GEOMETRY)
+    expect_error(tv, tv.lib().tdvh_sgm(ctx(tv), ct.byref(d32), ct.byref(p), None, dp_, W,
+                                       stream()), "reference geometry")
+    p = tv.sgm_params(fp1=-1.0)
+    df = dsi(tv, C, W, H, 8, tv.F32)
+    expect_error(tv, tv.lib().tdvh_sgm(ctx(tv), ct.byref(df), ct.byref(p), None, dp_, W,
+                                       stream()), "float penalties")
+    p = tv.sgm_params(p1=(1 << 30) + 1)
+    expect_error(tv, tv.lib().tdvh_sgm(ctx(tv), ct.byref(d32), ct.byref(p), None, dp_, W,
+                                       stream()), "penalties")
+    S16 = torch.zeros((H, W, 8), dtype=torch.int16, device=dev)
+    s16 = dsi(tv, S16, W, H, 8, tv.U16)
+    p = tv.sgm_params()
+    expect_error(tv, tv.lib().tdvh_sgm(ctx(tv), ct.byref(d32), ct.byref(p), ct.byref(s16), dp_,
+                                       W, stream()), "S_out")
+    still_works(tv, imgs)
+
+
+def test_batch_rejects_large_p1(tv, dev, imgs):
+    L, R = imgs
+    disp = torch.empty((H, W), dtype=torch.int16, device=dev)
+    with pytest.raises(tv.TdvhError, match="P1"):
+        tv.Batch([(L, R, D, disp)], p1=70000)
     still_works(tv, imgs)
 
 
@@ -170,4 +206,8 @@ def test_fast_wta_ssd_bad_output(tv, dev, imgs):
     rc = tv.lib().tdvh_fast_wta_ssd(ctx(tv), ct.c_void_p(L.data_ptr()), ct.c_void_p(R.data_ptr()),
                                     W, H, W, D, 3, ct.c_void_p(disp.data_ptr()), W - 1, stream())
     expect_error(tv, rc, "disparity output")
+    # the fused matchers keep one pass of at most 256 disparities
+    rc = tv.lib().tdvh_fast_wta_ssd(ctx(tv), ct.c_void_p(L.data_ptr()), ct.c_void_p(R.data_ptr()),
+                                    W, H, W, 257, 3, ct.c_void_p(disp.data_ptr()), W, stream())
+    expect_error(tv, rc, "depth")
     still_works(tv, imgs)

@@ -1,0 +1,208 @@
+"""GPU parity of every matching-cost x optimiser pairing the reference's
+factory composes (commonstereomatcherfactory.cpp:39-81): SGM over uint32
+(SSD) and float (NCC) volumes, DP over uint16 (BT) and float volumes, the
+bounded 32-bit DP's contract check and 64-bit recomputation, disparity
+ranges above 256 for the costs and the DP, and the float d/D view.
+Integer paths bit-exact; float paths bit-exact against the oracle's
+single-precision restatement with the same operation order."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from thundervision_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def to_dev(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def pair(W, H, dmax, seed):
+    L, R, _ = synth.pair(W, H, dmax, 8, seed)
+    return L, R
+
+
+def u16_to_u32(C):
+    return np.where(C == oracle.INV16, np.uint32(oracle.INV32), C.astype(np.uint32))
+
+
+# ---- SGM over uint32 volumes ---------------------------------------------------
+
+@pytest.mark.parametrize("W,H,D,r,mask", [(40, 23, 16, 3, 0xFF), (70, 31, 64, 3, 0xFF),
+                                          (37, 50, 100, 2, 0x5A), (64, 20, 256, 3, 0xFF),
+                                          (9, 13, 1, 1, 0xFF), (25, 19, 33, 3, 0x81)])
+def test_sgm_over_ssd_volume(tv, dev, W, H, D, r, mask):
+    L, R = pair(W, H, D - 1, 11 * W + D)
+    C = tv.cost_ssd(to_dev(L, dev), to_dev(R, dev), D, r)
+    disp, S = tv.sgm(C, p1=8, p2=32, dir_mask=mask, return_S=True)
+    want_S = oracle.sgm_u32(oracle.cost_ssd(L, R, D, r), 8, 32, mask)
+    assert (S.cpu().numpy().view(np.uint32) == want_S).all()
+    assert (disp.cpu().numpy() == oracle.wta(want_S)).all()
+
+
+def test_sgm_u32_invalid_holes_restarts_and_saturation(tv, dev):
+    rng = np.random.default_rng(21)
+    H, W, D = 17, 29, 70
+    C = rng.integers(0, 1 << 20, (H, W, D)).astype(np.uint32)
+    C[rng.random(C.shape) < 0.05] = oracle.INV32
+    C[3, 7, :] = oracle.INV32          # a pixel with no valid cell: paths restart
+    C[:, 11, :] = oracle.INV32         # a whole invalid column
+    C[9, 20, :5] = 0xFFFFFF00          # sums saturate to invalid
+    for p1, p2 in ((8, 32), (1000, 1 << 24)):
+        disp, S = tv.sgm(to_dev(C.view(np.int32), dev), p1=p1, p2=p2, return_S=True)
+        want = oracle.sgm_u32(C, p1, p2)
+        assert (S.cpu().numpy().view(np.uint32) == want).all()
+        assert (disp.cpu().numpy() == oracle.wta(want)).all()
+
+
+# ---- SGM over float volumes ------------------------------------------------------
+
+@pytest.mark.parametrize("W,H,D,r,zm,mask", [(45, 30, 16, 4, True, 0xFF),
+                                             (64, 40, 100, 1, False, 0xFF),
+                                             (33, 33, 256, 4, True, 0x3C),
+                                             (50, 20, 32, 2, False, 0x81)])
+def test_sgm_over_ncc_volume(tv, dev, W, H, D, r, zm, mask):
+    L, R = pair(W, H, D - 1, 5 * W + r)
+    C = tv.cost_ncc(to_dev(L, dev), to_dev(R, dev), D, r, zm)
+    fp1, fp2 = 30.0 / 255, 150.0 / 255
+    disp, S = tv.sgm(C, dir_mask=mask, fp1=fp1, fp2=fp2, return_S=True)
+    want = oracle.sgm_f32(C.cpu().numpy(), np.float32(fp1), np.float32(fp2), mask)
+    got = S.cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert (disp.cpu().numpy() == oracle.wta(want)).all()
+
+
+def test_sgm_f32_nan_inf_and_flat_windows(tv, dev):
+    rng = np.random.default_rng(8)
+    H, W, D = 15, 31, 40
+    C = (rng.random((H, W, D)) * 2).astype(np.float32)
+    C[rng.random(C.shape) < 0.04] = np.inf
+    C[rng.random(C.shape) < 0.01] = np.nan
+    C[4, 9, :] = np.inf  # a pixel with no valid cell (a flat NCC window)
+    disp, S = tv.sgm(to_dev(C, dev), fp1=0.1, fp2=0.7, return_S=True)
+    want = oracle.sgm_f32(C, np.float32(0.1), np.float32(0.7))
+    assert np.array_equal(S.cpu().numpy().view(np.uint32), want.view(np.uint32))
+    assert (disp.cpu().numpy() == oracle.wta(want)).all()
+
+
+# ---- uint16 SGM: a predecessor with no valid cell restarts the path ----------------
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_sgm_u16_restarts_after_all_invalid_pixel(tv, dev, layout):
+    L, R = pair(48, 20, 15, 99)
+    C = oracle.cost_bt2(L, R, 16)
+    C[5, 30, :] = oracle.INV16
+    C[12, :, :] = np.where(np.arange(48)[:, None] % 7 == 3, oracle.INV16, C[12])
+    want = oracle.sgm_std(C, 8, 32)
+    if layout == 0:
+        disp, S = tv.sgm(to_dev(C.view(np.int16), dev), return_S=True)
+    else:
+        Cs = tv.new_split_volume(20, 48, 16, dev)
+        src = tv.dsi_for(to_dev(C.view(np.int16), dev), 16, tv.U16)
+        dst = tv.dsi_for(Cs, 16, tv.U16, tv.LAYOUT_SPLIT16)
+        tv._check(tv.lib().tdvh_dsi_convert(tv._ctx(None), tv.ct.byref(src), tv.ct.byref(dst),
+                                            tv._stream()))
+        disp, S = tv.sgm(Cs, depth=16, layout=tv.LAYOUT_SPLIT16, return_S=True)
+    assert (S.cpu().numpy().view(np.uint16) == want).all()
+    assert (disp.cpu().numpy() == oracle.wta(want)).all()
+
+
+# ---- DP over uint16 and float volumes ----------------------------------------------
+
+@pytest.mark.parametrize("W,H,D,flags", [(90, 20, 16, 0), (130, 17, 64, 2), (70, 9, 130, 1),
+                                         (300, 6, 256, 0)])
+def test_dp_over_bt_volume(tv, dev, W, H, D, flags):
+    L, R = pair(W, H, D - 1, W * 3 + D)
+    C = tv.cost_bt(to_dev(L, dev), to_dev(R, dev), D)
+    got = tv.dp(C, p_occ=20, flags=flags).cpu().numpy()
+    assert (got == oracle.dp_u16(oracle.cost_bt2(L, R, D), 20, flags)).all()
+
+
+def test_dp_u16_contract_break_recomputed_in_64_bits(tv, dev):
+    # invalid cells away from d > x break the 32-bit form's contract: those
+    # rows are recomputed with 64-bit sums; the others stay 32-bit
+    L, R = pair(120, 12, 40, 5)
+    C = oracle.cost_bt2(L, R, 48)
+    C[3, 60, 10] = oracle.INV16
+    C[7, 2, 0] = oracle.INV16
+    C[9, 100, :] = oracle.INV16
+    got = tv.dp(to_dev(C.view(np.int16), dev), p_occ=20).cpu().numpy()
+    assert (got == oracle.dp_u16(C, 20)).all()
+
+
+@pytest.mark.parametrize("W,H,D,r,zm,pen", [(60, 20, 32, 4, True, 0.0), (90, 15, 100, 1, False, 0.25),
+                                            (40, 11, 256, 4, True, 0.05)])
+def test_dp_over_ncc_volume(tv, dev, W, H, D, r, zm, pen):
+    L, R = pair(W, H, D - 1, 9 * W + r)
+    C = tv.cost_ncc(to_dev(L, dev), to_dev(R, dev), D, r, zm)
+    got = tv.dp(C, p_occ=pen).cpu().numpy()
+    assert (got == oracle.dp_f32(C.cpu().numpy(), np.float32(pen))).all()
+
+
+def test_dp_f32_nan_inf(tv, dev):
+    rng = np.random.default_rng(3)
+    C = rng.random((7, 55, 20)).astype(np.float32)
+    C[rng.random(C.shape) < 0.05] = np.inf
+    C[rng.random(C.shape) < 0.02] = np.nan
+    for flags in (0, 1, 2):
+        got = tv.dp(to_dev(C, dev), p_occ=0.125, flags=flags).cpu().numpy()
+        assert (got == oracle.dp_f32(C, 0.125, flags)).all()
+
+
+# ---- bounded 32-bit DP: contract check and 64-bit recomputation ---------------------
+
+def test_dp_bounded_flag_with_out_of_contract_rows(tv, dev):
+    rng = np.random.default_rng(17)
+    H, W, D = 16, 200, 64
+    C = rng.integers(0, 1 << 21, (H, W, D)).astype(np.uint32)
+    xs = np.arange(W)[None, :, None]
+    ds = np.arange(D)[None, None, :]
+    C = np.where(ds > xs, np.uint32(oracle.INV32), C)
+    C[2, 150, 7] = 1 << 30             # a cost above 2^22
+    C[5, 10, 3] = oracle.INV32         # an invalid cell where d <= x
+    C[8, 20, 40] = 123                 # a valid cell where d > x
+    C[11, 199, 63] = (1 << 22)         # exactly the limit
+    for flags in (tv.DP_COST_BOUNDED, tv.DP_COST_BOUNDED | tv.DP_REF_BACKTRACK):
+        got = tv.dp(to_dev(C.view(np.int32), dev), p_occ=20, flags=flags).cpu().numpy()
+        assert (got == oracle.dp(C, 20, flags & ~tv.DP_COST_BOUNDED)).all()
+
+
+# ---- disparity ranges above 256 -----------------------------------------------------
+
+@pytest.mark.parametrize("W,H,D,r", [(330, 12, 300, 3), (530, 9, 512, 2), (80, 10, 257, 1)])
+def test_ssd_and_dp_above_256(tv, dev, W, H, D, r):
+    L, R = pair(W, H, D - 1, W + D)
+    want_C = oracle.cost_ssd(L, R, D, r)
+    C = tv.cost_ssd(to_dev(L, dev), to_dev(R, dev), D, r)
+    assert (C.cpu().numpy().view(np.uint32) == want_C).all()
+    assert (tv.wta(C).cpu().numpy() == oracle.wta(want_C)).all()
+    for flags in (0, tv.DP_COST_BOUNDED):
+        got = tv.dp(C, p_occ=20, flags=flags).cpu().numpy()
+        assert (got == oracle.dp(want_C, 20)).all()
+
+
+def test_ncc_and_bt_above_256(tv, dev):
+    W, H, D = 360, 10, 320
+    L, R = pair(W, H, D - 1, 77)
+    C64, rho = oracle.cost_ncc(L, R, D, 3, True)
+    got = tv.cost_ncc(to_dev(L, dev), to_dev(R, dev), D, 3, True).cpu().numpy()
+    fin = np.isfinite(C64)
+    assert (np.isfinite(got) == fin).all()
+    err = np.abs(got[fin].astype(np.float64) - C64[fin])
+    assert (err <= 1e-5 * np.maximum(np.abs(rho[fin]), 1e-2)).all()
+    Cb = tv.cost_bt(to_dev(L, dev), to_dev(R, dev), 400).cpu().numpy().view(np.uint16)
+    assert (Cb == oracle.cost_bt2(L, R, 400)).all()
+    dpb = tv.dp(tv.cost_bt(to_dev(L, dev), to_dev(R, dev), 400), p_occ=20).cpu().numpy()
+    assert (dpb == oracle.dp_u16(oracle.cost_bt2(L, R, 400), 20)).all()
+
+
+# ---- float d/D view --------------------------------------------------------------------
+
+@pytest.mark.parametrize("D", [37, 100, 192, 64, 509])
+def test_disp_to_unit_float_is_the_ieee_quotient(tv, dev, D):
+    d = np.arange(D, dtype=np.int16).reshape(1, D)
+    got = tv.disp_to_unit_float(to_dev(d, dev), D).cpu().numpy()
+    want = d.astype(np.float32) / np.float32(D)  # one IEEE single division
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))

@@ -164,10 +164,13 @@ def test_dp_bounded_extreme_costs(tv, dev, seed):
     assert (tv.dp(Cx, p_occ, tv.DP_COST_BOUNDED).cpu().numpy() == want).all()
 
 
-def test_dp_bounded_rejects_large_penalty(tv, dev):
-    C = to_dev(np.zeros((2, 8, 4), np.int32), dev)
-    with pytest.raises(RuntimeError, match="p_occ"):
-        tv.dp(C, 1 << 20, tv.DP_COST_BOUNDED)
+def test_dp_bounded_flag_with_large_penalty_runs_64_bit(tv, dev):
+    # the 32-bit form needs p_occ < 2^20; above it the flag is ignored
+    rng = np.random.default_rng(12)
+    C = rng.integers(0, 1 << 21, (3, 40, 9)).astype(np.uint32)
+    for p in ((1 << 20), (1 << 31) + 5):
+        got = tv.dp(to_dev(C.view(np.int32), dev), p, tv.DP_COST_BOUNDED).cpu().numpy()
+        assert (got == oracle.dp(C, p)).all()
 
 
 @pytest.mark.parametrize("hi,inv", [(1 << 20, 0.0), (1 << 31, 0.05), (0xFFFFFFF0, 0.3)])

@@ -306,3 +306,117 @@ def test_generator_known_answers():
 def test_fast_wta_equals_ssd_then_wta():
     L, R = rand_img(10, 16), rand_img(10, 16)
     assert (oracle.fast_wta_ssd(L, R, 5) == oracle.wta(oracle.cost_ssd(L, R, 5))).all()
+
+
+# ---- reference-geometry start-assign: closed form -----------------------------
+
+@pytest.mark.parametrize("W,H", [(12, 1), (1, 11), (17, 1)])
+def test_ref_start_assign_one_row_or_column_closed_form(W, H):
+    # On a single row (or column) every column and diagonal path of the
+    # reference table (semiglobal.cu:116-230) is one pixel long and comes
+    # after the row (column) path in table order; with start-assign
+    # (semiglobalcpu.cpp:81) the last traversal starting at each pixel sets
+    # S = C there and nothing is added afterwards, so S = C exactly.
+    C = oracle.cost_bt2(rand_img(H, W), rand_img(H, W), 5)
+    S = oracle.sgm_ref(C, 8, 32, flags=oracle.SGM_START_ASSIGN)
+    v = C != oracle.INV16
+    assert (S[v] == C[v].astype(np.int32)).all() and (S[~v] == oracle.INF_I32).all()
+    # without it each pixel sums the two traversals of the long path and
+    # one C per one-pixel traversal over it (the table's W != H end points
+    # pile some of them onto the end pixels; the visit counts are pinned by
+    # test_reference_path_coverage's closed forms)
+    S0 = oracle.sgm_ref(C, 8, 32, flags=0)
+    long_dir = (0, 1) if H == 1 else (2, 3)
+    Lsum = sum(oracle.sgm_lr(C, 8, 32, d).astype(np.int64) for d in long_dir)
+    visits = oracle.sgm_ref_visits(W, H)
+    short = sum(visits[d] for d in range(8) if d not in long_dir)[:, :, None]
+    assert (S0[v] == (Lsum + short * C.astype(np.int64))[v]).all()
+
+
+# ---- SGM restart after a predecessor with no valid cell ------------------------
+
+def test_sgm_restarts_after_an_all_invalid_pixel():
+    # one row, direction (1,0): pixel k has no valid cell, so the path
+    # restarts at k + 1 exactly as if the row began there (L = C, then the
+    # textbook recurrence), in the uint16, uint32 and float forms alike
+    W, D, k = 9, 4, 4
+    C = rng.integers(0, 80, (1, W, D)).astype(np.uint16)
+    C[0, k, :] = oracle.INV16
+    Lr = oracle.sgm_lr(C, 5, 17, 0)
+    tail = oracle.sgm_lr(np.ascontiguousarray(C[:, k + 1:]), 5, 17, 0)
+    assert (Lr[0, k] == oracle.INV16).all()
+    assert (Lr[:, k + 1:] == tail).all()
+    C32 = np.where(C == oracle.INV16, np.uint32(oracle.INV32), C.astype(np.uint32))
+    S32 = oracle.sgm_u32(C32, 5, 17, dir_mask=1)
+    assert (S32[:, k + 1:] == tail.astype(np.uint32)).all()
+    Cf = np.where(C == oracle.INV16, np.float32(np.inf), C.astype(np.float32))
+    Sf = oracle.sgm_f32(Cf, 5.0, 17.0, dir_mask=1)
+    assert (Sf[:, k + 1:] == tail.astype(np.float32)).all() and np.isinf(Sf[0, k]).all()
+
+
+# ---- SGM over uint32 and float volumes -------------------------------------------
+
+def _u16_to_u32(C):
+    return np.where(C == oracle.INV16, np.uint32(oracle.INV32), C.astype(np.uint32))
+
+
+@pytest.mark.parametrize("mask", [0xFF, 0x01, 0x5A])
+def test_sgm_u32_equals_uint16_sgm_on_its_domain(mask):
+    # the uint32 form (saturating, never reached here) and the pinned
+    # uint16 form are separate code; on uint16 volumes they must agree
+    H, W, D = 9, 14, 7
+    C = oracle.cost_bt2(rand_img(H, W), rand_img(H, W), D)
+    C[2, 5, :] = oracle.INV16  # a pixel with no valid cell
+    S16 = oracle.sgm_std(C, 8, 32, mask)
+    S32 = oracle.sgm_u32(_u16_to_u32(C), 8, 32, mask)
+    assert (S32 == _u16_to_u32(S16)).all()
+
+
+def test_sgm_u32_zero_penalties_and_saturation():
+    C = rng.integers(0, 1 << 28, (5, 8, 6)).astype(np.uint32)
+    S = oracle.sgm_u32(C, 0, 0)
+    assert (S.astype(np.uint64) == 8 * C.astype(np.uint64)).all()
+    # costs near 2^32: every sum saturates at the invalid value
+    big = np.full((3, 4, 3), 0xF0000000, np.uint32)
+    assert (oracle.sgm_u32(big, 0, 0) == oracle.INV32).all()
+
+
+def test_sgm_f32_dyadic_costs_equal_integer_sgm():
+    # costs and penalties k/8 with small k: every float operation is exact,
+    # so the float form is the integer form divided by 8
+    H, W, D = 8, 11, 6
+    C = oracle.cost_bt2(rand_img(H, W), rand_img(H, W), D)
+    Cf = np.where(C == oracle.INV16, np.float32(np.inf), C.astype(np.float32) / 8)
+    for mask in (0xFF, 0x81):
+        S16 = oracle.sgm_std(C, 8, 32, mask)
+        Sf = oracle.sgm_f32(Cf, 1.0, 4.0, mask)
+        v = C != oracle.INV16
+        assert (Sf[v] == S16[v].astype(np.float32) / 8).all() and np.isinf(Sf[~v]).all()
+
+
+def test_sgm_f32_nan_and_negative_infinity_read_as_invalid():
+    C = rng.random((4, 6, 5)).astype(np.float32)
+    Cn = C.copy()
+    Cn[1, 2, 3] = np.nan
+    Cn[2, 4, 0] = -np.inf
+    Ci = C.copy()
+    Ci[1, 2, 3] = np.inf
+    Ci[2, 4, 0] = np.inf
+    a = oracle.sgm_f32(Cn, 0.1, 0.5)
+    b = oracle.sgm_f32(Ci, 0.1, 0.5)
+    assert np.array_equal(a, b)
+
+
+# ---- DP over uint16 and float volumes ------------------------------------------
+
+def test_dp_f32_dyadic_costs_equal_integer_dp():
+    C = rng.integers(0, 400, (4, 23, 6)).astype(np.uint32)
+    C[:, 0, 1:] = oracle.INV32
+    Cf = np.where(C == oracle.INV32, np.float32(np.inf), C.astype(np.float32) / 8)
+    for flags in (0, oracle.DP_START_ZERO, oracle.DP_REF_BACKTRACK):
+        assert (oracle.dp_f32(Cf, 20 / 8, flags) == oracle.dp(C, 20, flags)).all()
+
+
+def test_dp_u16_reads_inv16_as_invalid():
+    C = oracle.cost_bt2(rand_img(5, 17), rand_img(5, 17), 6)
+    assert (oracle.dp_u16(C, 20) == oracle.dp(_u16_to_u32(C), 20)).all()

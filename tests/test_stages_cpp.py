@@ -47,4 +47,26 @@ def test_stage_chains_on_gpu(tmp_path):
     Sf = oracle.cost_ssd(Lf, Rf, D, 3)
     oracle.dp(Sf, 20).tofile(tmp_path / "ssd_dp_pf.i16")
     oracle.wta(Sf).tofile(tmp_path / "ssd_wta_pf.i16")
+    # all nine factory pairings on the prefiltered pair; the NCC volume
+    # (CrossCorrelationDev's 3x3 plain NCC) comes from the device, whose
+    # float costs are pinned against the FP64 oracle elsewhere, and the
+    # optimisers' float arithmetic is checked bit-exact against the oracle's
+    import torch
+    import thundervision_amd.tdvh as tv
+    dev = torch.device("cuda:0")
+    Nf = tv.cost_ncc(torch.from_numpy(Lf).to(dev), torch.from_numpy(Rf).to(dev), D, 1,
+                     False).cpu().numpy()
+    fp1, fp2 = np.float32(30.0 / 255), np.float32(150.0 / 255)
+    vols = {"ssd": (Sf, lambda: oracle.dp(Sf, 20), lambda: oracle.sgm_u32(Sf, 8, 32)),
+            "ncc": (Nf, lambda: oracle.dp_f32(Nf, 0.0), lambda: oracle.sgm_f32(Nf, fp1, fp2)),
+            "bt": (Cf, lambda: oracle.dp_u16(Cf, 20), lambda: oracle.sgm_std(Cf, 8, 32))}
+    for name, (vol, dpf, sgmf) in vols.items():
+        oracle.wta(vol).tofile(tmp_path / ("pair_%s_wta.i16" % name))
+        dpf().tofile(tmp_path / ("pair_%s_dp.i16" % name))
+        oracle.wta(sgmf()).tofile(tmp_path / ("pair_%s_sgm.i16" % name))
+    # TestSemiGlobalDev.WithSSD / TestSemiGlobalCPU.WithSSD: SSD -> SGM at 64
+    # and 256 disparities
+    for Dd in (64, 256):
+        oracle.wta(oracle.sgm_u32(oracle.cost_ssd(Lf, Rf, Dd, 3), 8, 32)).tofile(
+            tmp_path / ("ssd_sgm_pf_%d.i16" % Dd))
     assert "gpu: ok" in run("gpu", str(tmp_path))

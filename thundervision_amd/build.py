@@ -16,7 +16,7 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 INCLUDE = os.path.join(ROOT, "include")
 
-HIP_SOURCES = ["ctx.hip", "cost_bt.hip", "cost_window.hip", "dp.hip", "sgm.hip", "sgm_ref.hip",
+HIP_SOURCES = ["ctx.hip", "cost_bt.hip", "cost_window.hip", "dp.hip", "sgm.hip", "sgm_ref.hip", "sgm_wide.hip",
                "wta.hip", "batch.hip", "io.hip", "pipeline.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [

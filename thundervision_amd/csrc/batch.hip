@@ -181,7 +181,9 @@ extern "C" int tdvh_batch_create(tdvh_ctx *ctx, const tdvh_pair *pairs, int n,
     TDVH_REQUIRE(pairs && out && n > 0 && params, "bad arguments");
     TDVH_REQUIRE(params->dir_mask == 0xFF && params->flags == 0,
                  "batches run the standard 8-direction geometry");
-    TDVH_REQUIRE(params->p1 >= 0 && params->p2 >= 0 && params->p2 <= 255,
+    TDVH_REQUIRE(params->p1 >= 0 && params->p1 <= 1024,
+                 "P1 %d outside [0, 1024] (the per-pair limit)", params->p1);
+    TDVH_REQUIRE(params->p2 >= 0 && params->p2 <= 255,
                  "batches need 0 <= P2 <= 255 (byte path-cost planes)");
     std::vector<BPair> bp(n);
     std::vector<size_t> off_terms(n), off_planes(n);

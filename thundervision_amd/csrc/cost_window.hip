@@ -49,6 +49,7 @@ constexpr int WIN_TY = 32; // output rows per wave
 struct WinArgs {
     const uint8_t *L, *R;
     int W, H, pitch, D;
+    int d0;             // first disparity of this pass (volumes above 256 run in passes)
     int64_t sx, sy;     // output strides (elements)
     void *out;          // uint32 / float volume, or int16 disparities (WTA)
     int disp_pitch;
@@ -109,7 +110,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void wi
     const int x0 = blockIdx.x * WIN_TX, y0 = blockIdx.y * WIN_TY;
     const int W = a.W, H = a.H;
     const int y1 = min(y0 + WIN_TY, H); // one past the last output row
-    const int lbase = x0 - RAD, rbase = x0 - RAD - (NCH * 64 - 1);
+    const int lbase = x0 - RAD, rbase = x0 - RAD - a.d0 - (NCH * 64 - 1);
 
     // fetch image row v (clamped) into registers: word k of the slot
     uint32_t pre[PER_LANE];
@@ -174,7 +175,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void wi
 #pragma unroll
                 for (int q = 0; q < B_PER_LANE; q++) {
                     const int k = lane + 64 * q;
-                    const int u = x0 - (NCH * 64 - 1) + k; // only 0 <= u < W are read later
+                    const int u = x0 - a.d0 - (NCH * 64 - 1) + k; // only 0 <= u < W are read later
                     const size_t o = row + (u < 0 ? 0 : (u >= W ? W - 1 : u));
                     bpre[q] = a.box[2 * pl + o];
                     // x - d < 0: a NaN reciprocal root makes the cost
@@ -231,12 +232,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void wi
             // x - d = rem + t - 64 c < 0 sets it through the sign, and so does
             // d >= D in the last chunk; rem is hidden from the compiler per
             // row so that nothing is hoisted out of the row loop
-            int rem = x0 - lane;
+            int rem = x0 - a.d0 - lane;
             asm volatile("" : "+v"(rem));
-            const uint32_t dbad = (uint32_t)((a.D - 1 - (NCH - 1) * 64 - lane) >> 31);
+            const uint32_t dbad = (uint32_t)((a.D - 1 - a.d0 - (NCH - 1) * 64 - lane) >> 31);
 #pragma unroll
             for (int c = 0; c < NCH; c++) {
-                const int d = c * 64 + lane;
+                const int d = a.d0 + c * 64 + lane;
                 uint32_t s = 0;
 #pragma unroll
                 for (int i = 0; i < NWIN; i++) s += col[c][i];
@@ -294,7 +295,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void wi
                     // NaN for an invalid cell, which no comparison lets win
                     const float cost = __fmaf_rn(-(float)num, __fmul_rn(ql, sqr[k]), 1.0f);
                     // only the last chunk can hold d >= D (never a minimum)
-                    const bool better = cost < best[t] && (c < NCH - 1 || d < a.D);
+                    const bool better = cost < best[t] && (c < NCH - 1 || a.d0 + d < a.D);
                     best[t] = better ? cost : best[t];
                     // chunk 0 leaves the column's bits at their initial 0
                     if (c > 0)
@@ -309,7 +310,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void wi
 #pragma unroll
             for (int t = 0; t < WIN_TX; t++) {
                 const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)kmin, 2 * t);
-                key[t] = key[t] == m ? ((bcs >> (2 * t)) & 3u) * 64 + (uint32_t)lane : 0xFFFFu;
+                key[t] = key[t] == m ? a.d0 + ((bcs >> (2 * t)) & 3u) * 64 + (uint32_t)lane : 0xFFFFu;
             }
             const uint32_t dmin = wave_min16(key, lane);
             const int t = (lane >> 1) & 15;
@@ -333,7 +334,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void wi
 #pragma unroll
                 for (int i = 0; i < NWIN; i++) sum[c] += col[c][i];
             }
-            char *p = (char *)a.out + ((int64_t)x0 * a.sx + (int64_t)y * a.sy + lane) * 4;
+            char *p = (char *)a.out + ((int64_t)x0 * a.sx + (int64_t)y * a.sy + a.d0 + lane) * 4;
             const int64_t px = a.sx * 4;
 #pragma unroll
             for (int t = 0; t < WIN_TX; t++) {
@@ -359,7 +360,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void wi
                     const float inv = __fmul_rn(ql, qr);
                     const float cost = __builtin_fminf(__fmaf_rn(-(float)num, inv, 1.0f), pinf);
                     // only the last chunk can hold d >= D
-                    if (c < NCH - 1 || d < a.D) __builtin_nontemporal_store(cost, (float *)(p + t * px + c * 256));
+                    if (c < NCH - 1 || a.d0 + d < a.D) __builtin_nontemporal_store(cost, (float *)(p + t * px + c * 256));
                 }
             }
         } else {
@@ -374,12 +375,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void wi
 #pragma unroll
                 for (int i = 0; i < NWIN; i++) sum[c] += col[c][i];
             }
-            char *p = (char *)a.out + ((int64_t)x0 * a.sx + (int64_t)y * a.sy + lane) * 4;
+            char *p = (char *)a.out + ((int64_t)x0 * a.sx + (int64_t)y * a.sy + a.d0 + lane) * 4;
             const int64_t px = a.sx * 4;
             // x - d = rem + t - 64 c; hidden from the compiler per row, which
             // would otherwise hoist every (t, c) comparison out of the row
             // loop into spilled scalar masks
-            int rem = x0 - lane;
+            int rem = x0 - a.d0 - lane;
             asm volatile("" : "+v"(rem));
 #pragma unroll
             for (int t = 0; t < WIN_TX; t++) {
@@ -392,7 +393,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void wi
                 for (int c = 0; c < NCH; c++) {
                     const uint32_t v = rem + t - 64 * c >= 0 ? sum[c] : INV32;
                     // only the last chunk can hold d >= D
-                    if (c < NCH - 1 || c * 64 + lane < a.D)
+                    if (c < NCH - 1 || a.d0 + c * 64 + lane < a.D)
                         __builtin_nontemporal_store(v, (uint32_t *)(p + t * px + c * 256));
                 }
             }
@@ -454,24 +455,30 @@ __global__ __launch_bounds__(256) void box_sums_kernel(const uint8_t *__restrict
                               : __builtin_bit_cast(int32_t, __fdiv_rn(1.0f, __fsqrt_rn((float)vr)));
 }
 
+// Volumes run in passes of at most 256 disparities (4 chunks per lane);
+// the fused WTA modes take D <= 256 in one pass.
 template <int RAD, int MODE>
-static int launch_window_r(const WinArgs &a, hipStream_t s)
+static int launch_window_r(const WinArgs &args, hipStream_t s)
 {
-    dim3 grid((a.W + WIN_TX - 1) / WIN_TX, (a.H + WIN_TY - 1) / WIN_TY);
-    const int nch = (a.D + 63) / 64;
-    prof_begin(s);
-    switch (nch) {
-    case 1: window_kernel<RAD, 1, MODE><<<grid, 64, 0, s>>>(a); break;
-    case 2: window_kernel<RAD, 2, MODE><<<grid, 64, 0, s>>>(a); break;
-    case 3: window_kernel<RAD, 3, MODE><<<grid, 64, 0, s>>>(a); break;
-    case 4: window_kernel<RAD, 4, MODE><<<grid, 64, 0, s>>>(a); break;
-    default: set_error("window costs support D <= 256 (got %d)", a.D); return TDVH_ERR_UNSUPPORTED;
+    dim3 grid((args.W + WIN_TX - 1) / WIN_TX, (args.H + WIN_TY - 1) / WIN_TY);
+    for (int d0 = 0; d0 < args.D; d0 += 256) {
+        WinArgs a = args;
+        a.d0 = d0;
+        const int nch = (args.D - d0 + 63) / 64;
+        prof_begin(s);
+        switch (nch) {
+        case 1: window_kernel<RAD, 1, MODE><<<grid, 64, 0, s>>>(a); break;
+        case 2: window_kernel<RAD, 2, MODE><<<grid, 64, 0, s>>>(a); break;
+        case 3: window_kernel<RAD, 3, MODE><<<grid, 64, 0, s>>>(a); break;
+        default: window_kernel<RAD, 4, MODE><<<grid, 64, 0, s>>>(a); break;
+        }
+        TDVH_TRY(check_launch(MODE == WIN_NCC       ? "ncc_window_kernel"
+                              : MODE == WIN_SSD     ? "ssd_window_kernel"
+                              : MODE == WIN_SSD_WTA ? "ssd_wta_window_kernel"
+                                                    : "ncc_wta_window_kernel",
+                              s));
     }
-    return check_launch(MODE == WIN_NCC       ? "ncc_window_kernel"
-                        : MODE == WIN_SSD     ? "ssd_window_kernel"
-                        : MODE == WIN_SSD_WTA ? "ssd_wta_window_kernel"
-                                              : "ncc_wta_window_kernel",
-                        s);
+    return TDVH_OK;
 }
 
 template <int MODE>
@@ -487,11 +494,16 @@ static int launch_window(const WinArgs &a, int radius, hipStream_t s)
     }
 }
 
-static int check_images(const uint8_t *l, const uint8_t *r, int W, int H, int pitch, int D)
+// Cost volumes take D <= 512 (the reference's window costs loop over any D,
+// ssd.cu:41-45, crosscorrelation.cu:38-46); the fused WTA modes D <= 256.
+constexpr int WIN_MAX_DEPTH = 512, WIN_WTA_MAX_DEPTH = 256;
+
+static int check_images(const uint8_t *l, const uint8_t *r, int W, int H, int pitch, int D,
+                        int maxD)
 {
     TDVH_REQUIRE(l && r, "null image");
     TDVH_REQUIRE(W > 0 && H > 0 && pitch >= W, "bad image %dx%d pitch %d", W, H, pitch);
-    TDVH_REQUIRE(D > 0 && D <= 256, "depth %d outside [1, 256]", D);
+    TDVH_REQUIRE(D > 0 && D <= maxD, "depth %d outside [1, %d]", D, maxD);
     return TDVH_OK;
 }
 
@@ -528,11 +540,11 @@ extern "C" int tdvh_cost_ssd(tdvh_ctx *ctx, const uint8_t *left, const uint8_t *
     clear_error();
     TDVH_TRY(use_device(ctx));
     TDVH_REQUIRE(out && out->ptr, "null output");
-    TDVH_TRY(check_images(left, right, width, height, pitch, out->depth));
+    TDVH_TRY(check_images(left, right, width, height, pitch, out->depth, WIN_MAX_DEPTH));
     TDVH_REQUIRE(out->width == width && out->height == height && out->dtype == TDVH_U32 &&
                      out->layout == TDVH_LAYOUT_STRIDED,
                  "SSD output must be a strided uint32 %dx%d volume", width, height);
-    WinArgs a{left, right, width, height, pitch, out->depth, out->stride_x, out->stride_y,
+    WinArgs a{left, right, width, height, pitch, out->depth, 0, out->stride_x, out->stride_y,
               out->ptr, 0, nullptr, 0};
     return launch_window<WIN_SSD>(a, radius, (hipStream_t)stream);
 }
@@ -543,9 +555,9 @@ extern "C" int tdvh_fast_wta_ssd(tdvh_ctx *ctx, const uint8_t *left, const uint8
 {
     clear_error();
     TDVH_TRY(use_device(ctx));
-    TDVH_TRY(check_images(left, right, width, height, pitch, depth));
+    TDVH_TRY(check_images(left, right, width, height, pitch, depth, WIN_WTA_MAX_DEPTH));
     TDVH_REQUIRE(disp && disp_pitch >= width, "bad disparity output");
-    WinArgs a{left, right, width, height, pitch, depth, 0, 0, disp, disp_pitch, nullptr, 0};
+    WinArgs a{left, right, width, height, pitch, depth, 0, 0, 0, disp, disp_pitch, nullptr, 0};
     return launch_window<WIN_SSD_WTA>(a, radius, (hipStream_t)stream);
 }
 
@@ -556,7 +568,7 @@ extern "C" int tdvh_cost_ncc(tdvh_ctx *ctx, const uint8_t *left, const uint8_t *
     clear_error();
     TDVH_TRY(use_device(ctx));
     TDVH_REQUIRE(out && out->ptr, "null output");
-    TDVH_TRY(check_images(left, right, width, height, pitch, out->depth));
+    TDVH_TRY(check_images(left, right, width, height, pitch, out->depth, WIN_MAX_DEPTH));
     TDVH_REQUIRE(out->width == width && out->height == height && out->dtype == TDVH_F32 &&
                      out->layout == TDVH_LAYOUT_STRIDED,
                  "NCC output must be a strided float %dx%d volume", width, height);
@@ -564,7 +576,7 @@ extern "C" int tdvh_cost_ncc(tdvh_ctx *ctx, const uint8_t *left, const uint8_t *
     hipStream_t s = (hipStream_t)stream;
     const int32_t *box = nullptr;
     TDVH_TRY(ncc_box_sums(ctx, left, right, width, height, pitch, radius, zero_mean, &box, s));
-    WinArgs a{left, right, width, height, pitch, out->depth, out->stride_x, out->stride_y,
+    WinArgs a{left, right, width, height, pitch, out->depth, 0, out->stride_x, out->stride_y,
               out->ptr, 0, box, zero_mean};
     return launch_window<WIN_NCC>(a, radius, s);
 }
@@ -575,12 +587,12 @@ extern "C" int tdvh_fast_wta_ncc(tdvh_ctx *ctx, const uint8_t *left, const uint8
 {
     clear_error();
     TDVH_TRY(use_device(ctx));
-    TDVH_TRY(check_images(left, right, width, height, pitch, depth));
+    TDVH_TRY(check_images(left, right, width, height, pitch, depth, WIN_WTA_MAX_DEPTH));
     TDVH_REQUIRE(disp && disp_pitch >= width, "bad disparity output");
     TDVH_REQUIRE(radius >= 1 && radius <= 5, "window radius %d outside [1, 5]", radius);
     hipStream_t s = (hipStream_t)stream;
     const int32_t *box = nullptr;
     TDVH_TRY(ncc_box_sums(ctx, left, right, width, height, pitch, radius, zero_mean, &box, s));
-    WinArgs a{left, right, width, height, pitch, depth, 0, 0, disp, disp_pitch, box, zero_mean};
+    WinArgs a{left, right, width, height, pitch, depth, 0, 0, 0, disp, disp_pitch, box, zero_mean};
     return launch_window<WIN_NCC_WTA>(a, radius, s);
 }

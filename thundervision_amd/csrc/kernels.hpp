@@ -21,6 +21,10 @@ int launch_bt_planes(const uint8_t *L, const uint8_t *R, int W, int H, int pitch
 int run_sgm_ref(tdvh_ctx *ctx, const uint32_t *C_split, int W, int H, int D, int P1, int P2,
                 int flags, int32_t *S, hipStream_t s);
 
+// SGM over a strided uint32 or float volume into S (same dtype, strided),
+// directions of p->dir_mask in increasing index (sgm_wide.hip).
+int run_sgm_wide(const tdvh_dsi *C, const tdvh_sgm_params *p, const tdvh_dsi *S, hipStream_t s);
+
 int popcount8(int mask);
 
 // Frees the reference-geometry tables cached for a context.

@@ -342,14 +342,46 @@ int launch_sgm_sum_wta(const SgmArgs &a, int nplanes, bool eout, uint32_t *S_spl
 
 using namespace tdvh;
 
-static int validate_sgm_params(const tdvh_sgm_params *p)
+// Penalties of the cost volume's dtype and the direction mask.
+static int validate_sgm_params(const tdvh_sgm_params *p, int dtype = TDVH_U16)
 {
     TDVH_REQUIRE(p != nullptr, "params is null");
-    TDVH_REQUIRE(p->p1 >= 0 && p->p2 >= 0 && p->p1 <= 1024 && p->p2 <= 1024,
-                 "penalties out of range (P1=%d, P2=%d)", p->p1, p->p2);
+    if (dtype == TDVH_F32) {
+        TDVH_REQUIRE(p->fp1 >= 0.0f && p->fp2 >= 0.0f && p->fp1 < __builtin_inff() &&
+                         p->fp2 < __builtin_inff(),
+                     "float penalties out of range (P1=%g, P2=%g)", (double)p->fp1,
+                     (double)p->fp2);
+    } else {
+        const int lim = dtype == TDVH_U32 ? (1 << 30) : 1024;
+        TDVH_REQUIRE(p->p1 >= 0 && p->p2 >= 0 && p->p1 <= lim && p->p2 <= lim,
+                     "penalties out of range (P1=%d, P2=%d, limit %d)", p->p1, p->p2, lim);
+    }
     TDVH_REQUIRE((p->dir_mask & 0xFF) != 0 && (p->dir_mask & ~0xFF) == 0, "bad dir_mask 0x%x",
                  p->dir_mask);
     return TDVH_OK;
+}
+
+// SGM + WTA over a strided uint32 or float volume (sgm_wide.hip).
+static int sgm_wide(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params *params,
+                    tdvh_dsi *S_out, int16_t *disp, int disp_pitch, void *stream)
+{
+    const int W = C->width, H = C->height, D = C->depth;
+    TDVH_REQUIRE(C->layout == TDVH_LAYOUT_STRIDED, "uint32 and float volumes must be strided");
+    TDVH_REQUIRE(params->flags == 0, "reference geometry needs a uint16 cost volume");
+    tdvh_dsi S;
+    if (S_out) {
+        TDVH_REQUIRE(S_out->ptr && S_out->width == W && S_out->height == H &&
+                         S_out->depth == D && S_out->dtype == C->dtype &&
+                         S_out->layout == TDVH_LAYOUT_STRIDED,
+                     "S_out must be a strided %dx%dx%d volume of the cost dtype", W, H, D);
+        S = *S_out;
+    } else {
+        void *sb = nullptr;
+        TDVH_TRY(ctx->scratch("sgm_wide_s", (size_t)W * H * D * 4, &sb));
+        TDVH_TRY(tdvh_dsi_init(&S, sb, W, H, D, C->dtype, TDVH_LAYOUT_STRIDED));
+    }
+    TDVH_TRY(run_sgm_wide(C, params, &S, (hipStream_t)stream));
+    return tdvh_wta(ctx, &S, TDVH_WTA_LOWEST, disp, disp_pitch, stream);
 }
 
 // Standard-geometry pipeline from a SPLIT16 cost volume (Csplit) or from
@@ -428,13 +460,15 @@ extern "C" int tdvh_sgm(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params 
 {
     clear_error();
     TDVH_TRY(use_device(ctx));
-    TDVH_TRY(validate_sgm_params(params));
     TDVH_REQUIRE(C && C->ptr && disp, "null pointer argument");
-    TDVH_REQUIRE(C->dtype == TDVH_U16, "SGM needs a uint16 cost volume");
+    TDVH_REQUIRE(C->dtype == TDVH_U16 || C->dtype == TDVH_U32 || C->dtype == TDVH_F32,
+                 "SGM needs a uint16, uint32 or float cost volume (got dtype %d)", C->dtype);
+    TDVH_TRY(validate_sgm_params(params, C->dtype));
     const int W = C->width, H = C->height, D = C->depth;
     TDVH_REQUIRE(W > 0 && H > 0 && D > 0 && D <= MAX_DEPTH, "bad volume %dx%dx%d (D <= %d)", W, H,
                  D, MAX_DEPTH);
     TDVH_REQUIRE(disp_pitch >= W, "disp_pitch %d < width %d", disp_pitch, W);
+    if (C->dtype != TDVH_U16) return sgm_wide(ctx, C, params, S_out, disp, disp_pitch, stream);
     hipStream_t s = (hipStream_t)stream;
     const uint32_t *Csplit = (const uint32_t *)C->ptr;
     if (C->layout != TDVH_LAYOUT_SPLIT16) {

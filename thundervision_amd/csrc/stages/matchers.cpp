@@ -177,25 +177,25 @@ void GraphStereoMatcher::run(FloatImage left, FloatImage right, FloatImage out)
 
 StereoMatcher *CommonStereoMatcherFactory::createStereoMatcher()
 {
-    if (m_mode == CPU)
+    if (m_compMode == CPU)
         throw Exception("CPU (OpenCV) matchers are not part of this library; use Device");
     std::shared_ptr<MatchingCost> cost;
-    switch (m_cost) {
+    switch (m_matchMode) {
     case SSD: cost = std::make_shared<SSDDev>(m_maxDisparity); break;
     case CrossCorrelationNorm: cost = std::make_shared<CrossCorrelationDev>(m_maxDisparity); break;
     case BirchfieldTomasi: cost = std::make_shared<BirchfieldCostDev>(m_maxDisparity); break;
+    default: throw Exception("unknown matching cost mode");
     }
     std::shared_ptr<Optimizer> opt;
-    switch (m_opt) {
+    switch (m_optMode) {
     case WTA: opt = std::make_shared<WTADev>(); break;
-    case DynamicProg:
-        if (m_cost != SSD) throw Exception("DynamicProgDev runs on uint32 SSD costs");
-        opt = std::make_shared<DynamicProgDev>();
-        break;
-    case SemiGlobal:
-        if (m_cost != BirchfieldTomasi) throw Exception("SemiGlobalDev runs on uint16 BT costs");
-        opt = std::make_shared<SemiGlobalDev>();
-        break;
+    case WTAOnCPU: opt = std::make_shared<WTACPU>(); break;
+    case DynamicProg: opt = std::make_shared<DynamicProgDev>(); break;
+    case DynamicProgOnCPU: opt = std::make_shared<DynamicProgCPU>(); break;
+    case Global:
+    case SemiGlobal: opt = std::make_shared<SemiGlobalDev>(); break;
+    case SemiGlobalOnCPU: opt = std::make_shared<SemiGlobalCPU>(); break;
+    default: throw Exception("unknown optimisation mode");
     }
     DevStereoMatcher *m = new DevStereoMatcher(0);
     m->setMatchingCost(cost);

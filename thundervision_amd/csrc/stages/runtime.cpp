@@ -1,6 +1,8 @@
 // Runtime of the C++ stage layer: error checks, device processes, runner.
 #include <hip/hip_runtime.h>
 
+#include <exception>
+
 #include "tdvh.h"
 #include "tdvision/process.hpp"
 
@@ -19,18 +21,31 @@ void checkHip(int st, const char *what)
                               std::string(what) + ": " + hipGetErrorString((hipError_t)st));
 }
 
+// A unit that throws is dropped; its WriteGuard has finished its output
+// pipe on the way out, so the units downstream of it drain and finish
+// theirs, and a reader at the end of the chain sees the end instead of
+// waiting forever. The first exception is rethrown once the loop ends.
 void DeviceProcess::process()
 {
     checkHip(hipSetDevice(m_deviceId), "hipSetDevice");
     std::vector<bool> alive(m_units.size(), true);
+    std::exception_ptr err;
     bool cont = true;
     while (cont && !m_end) {
         cont = false;
         for (size_t i = 0; i < m_units.size(); i++) {
-            if (alive[i]) alive[i] = m_units[i]->update();
+            if (alive[i]) {
+                try {
+                    alive[i] = m_units[i]->update();
+                } catch (...) {
+                    if (!err) err = std::current_exception();
+                    alive[i] = false;
+                }
+            }
             cont = cont || alive[i];
         }
     }
+    if (err) std::rethrow_exception(err);
 }
 
 ProcessRunner::ProcessRunner(ProcessGroup &group, ExceptionReport *report)

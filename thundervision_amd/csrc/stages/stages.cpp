@@ -103,6 +103,12 @@ void WTADev::updateImpl(DSIMem dsi, FloatImage out)
 void DynamicProgDev::updateImpl(DSIMem dsi, FloatImage out)
 {
     tdvh_dsi c = dsi.mem();
+    if (c.dtype == TDVH_F32) {
+        checkTdvh(tdvh_dp_f32(deviceContext(), &c, m_poccf, m_flags, out.disparityDev(), c.width,
+                              nullptr),
+                  "tdvh_dp_f32");
+        return;
+    }
     int flags = m_flags;
     // costs below 2^22 from a known producer: 32-bit sums, same results
     const uint32_t bound = dsi.costBound();

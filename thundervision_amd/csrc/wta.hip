@@ -146,12 +146,15 @@ static int launch_wta(const tdvh_dsi *S, int mode, int16_t *disp, int pitch, hip
     return check_launch("wta_tree_kernel", s);
 }
 
+// float(d) / float(D) as one IEEE-rounded division (the value of
+// wtacpu.cpp:35-36 and wta.cu:83); a multiplication by a rounded 1/D would
+// differ by one ulp for many d when D is not a power of two.
 __global__ __launch_bounds__(256) void disp_to_unit_kernel(const int16_t *__restrict__ disp,
-                                                           int dpitch, int W, int H, float inv,
+                                                           int dpitch, int W, int H, float depth,
                                                            float *__restrict__ out, int opitch)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-    if (x < W) out[(size_t)y * opitch + x] = (float)disp[(size_t)y * dpitch + x] * inv;
+    if (x < W) out[(size_t)y * opitch + x] = __fdiv_rn((float)disp[(size_t)y * dpitch + x], depth);
 }
 
 __global__ __launch_bounds__(256) void unit_to_u8_kernel(const float *__restrict__ src, int spitch,
@@ -215,9 +218,8 @@ extern "C" int tdvh_disp_to_unit_float(tdvh_ctx *ctx, const int16_t *disp, int d
     dim3 grid((width + 255) / 256, height);
     hipStream_t s = (hipStream_t)stream;
     prof_begin(s);
-    disp_to_unit_kernel<<<grid, 256, 0, s>>>(disp, disp_pitch, width, height,
-                                                               1.0f / (float)depth, out,
-                                                               out_pitch);
+    disp_to_unit_kernel<<<grid, 256, 0, s>>>(disp, disp_pitch, width, height, (float)depth, out,
+                                             out_pitch);
     return check_launch("disp_to_unit_kernel", s);
 }
 

@@ -19,7 +19,8 @@ SGM_REF_ALL = SGM_REF_GEOMETRY | SGM_START_ASSIGN | SGM_MIN_SKIP_LAST
 WTA_LOWEST, WTA_TREE = 0, 1
 DP_START_ZERO, DP_REF_BACKTRACK, DP_COST_BOUNDED = 1, 2, 4
 FUSE_STAGED, FUSE_COST = 0, 1
-MAX_DEPTH = 256
+MAX_DEPTH = 256      # SGM (the reference's limit, semiglobal.cu:44)
+DP_MAX_DEPTH = 512   # DP and the matching costs
 
 
 class TdvhError(RuntimeError):
@@ -38,7 +39,7 @@ class DSI(ct.Structure):
 
 class SgmParams(ct.Structure):
     _fields_ = [("p1", ct.c_int32), ("p2", ct.c_int32), ("dir_mask", ct.c_int32),
-                ("flags", ct.c_int32)]
+                ("flags", ct.c_int32), ("fp1", ct.c_float), ("fp2", ct.c_float)]
 
 
 class Pair(ct.Structure):
@@ -75,6 +76,7 @@ SIGNATURES = {
                       _VP]),
     "tdvh_wta": (_I, [_VP, ct.POINTER(DSI), _I, _VP, _I, _VP]),
     "tdvh_dp": (_I, [_VP, ct.POINTER(DSI), _U32, _I, _VP, _I, _VP]),
+    "tdvh_dp_f32": (_I, [_VP, ct.POINTER(DSI), ct.c_float, _I, _VP, _I, _VP]),
     "tdvh_fast_wta_ssd": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, _I, _VP, _I, _VP]),
     "tdvh_fast_wta_ncc": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _VP, _I, _VP]),
     "tdvh_stereo_bt_sgm": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, ct.POINTER(SgmParams), _I, _VP,
@@ -259,22 +261,26 @@ def cost_ncc(left, right, depth, radius=4, zero_mean=True, ctx=None, stream=None
     return out
 
 
-def sgm_params(p1=8, p2=32, dir_mask=0xFF, flags=0):
-    return SgmParams(p1, p2, dir_mask, flags)
+def sgm_params(p1=8, p2=32, dir_mask=0xFF, flags=0, fp1=30.0 / 255, fp2=150.0 / 255):
+    return SgmParams(p1, p2, dir_mask, flags, fp1, fp2)
 
 
 def sgm(C, depth=None, p1=8, p2=32, dir_mask=0xFF, flags=0, return_S=False, layout=None,
-        ctx=None, stream=None):
-    """SGM + WTA over a uint16 cost volume: (H, W, D) int16-bits strided, or
-    SPLIT16 when layout == LAYOUT_SPLIT16. Returns disp (and S)."""
+        fp1=30.0 / 255, fp2=150.0 / 255, ctx=None, stream=None):
+    """SGM + WTA over a cost volume: (H, W, D) strided int16 (uint16 BT bits),
+    int32 (uint32 SSD bits) or float32 (NCC), or a SPLIT16 uint16 volume when
+    layout == LAYOUT_SPLIT16. Float volumes use the penalties fp1/fp2.
+    Returns disp (and S, in the cost's dtype; int32 in reference geometry)."""
     H, W = C.shape[0], C.shape[1]
     if layout is None:
         layout = LAYOUT_STRIDED
     if depth is None:
         assert layout == LAYOUT_STRIDED
         depth = C.shape[2]
-    cd = dsi_for(C, depth, U16, layout)
-    params = sgm_params(p1, p2, dir_mask, flags)
+    dt = U16 if layout == LAYOUT_SPLIT16 else {torch.int16: U16, torch.int32: U32,
+                                                torch.float32: F32}[C.dtype]
+    cd = dsi_for(C, depth, dt, layout)
+    params = sgm_params(p1, p2, dir_mask, flags, fp1, fp2)
     disp = torch.empty((H, W), dtype=torch.int16, device=C.device)
     S = None
     sp = None
@@ -283,8 +289,9 @@ def sgm(C, depth=None, p1=8, p2=32, dir_mask=0xFF, flags=0, return_S=False, layo
             S = torch.empty((H, W, depth), dtype=torch.int32, device=C.device)
             sd = dsi_for(S, depth, I32)
         else:
-            S = torch.empty((H, W, depth), dtype=torch.int16, device=C.device)
-            sd = dsi_for(S, depth, U16)
+            tdt = {U16: torch.int16, U32: torch.int32, F32: torch.float32}[dt]
+            S = torch.empty((H, W, depth), dtype=tdt, device=C.device)
+            sd = dsi_for(S, depth, dt)
         sp = ct.byref(sd)
     _check(lib().tdvh_sgm(_ctx(ctx), ct.byref(cd), ct.byref(params), sp, _ptr(disp), W,
                           _stream(stream)))
@@ -306,11 +313,17 @@ def wta_dsi(S, dsi, mode=WTA_LOWEST, ctx=None, stream=None):
 
 
 def dp(C, p_occ=20, flags=0, ctx=None, stream=None):
+    """Scanline DP over an (H, W, D) volume: int16 (uint16 BT bits), int32
+    (uint32 SSD bits) or float32 (NCC, float penalty p_occ)."""
     H, W, D = C.shape
-    cd = dsi_for(C, D, U32)
+    cd = dsi_for(C, D, {torch.int16: U16, torch.int32: U32, torch.float32: F32}[C.dtype])
     disp = torch.empty((H, W), dtype=torch.int16, device=C.device)
-    _check(lib().tdvh_dp(_ctx(ctx), ct.byref(cd), p_occ, flags, _ptr(disp), W,
-                         _stream(stream)))
+    if C.dtype == torch.float32:
+        _check(lib().tdvh_dp_f32(_ctx(ctx), ct.byref(cd), float(p_occ), flags, _ptr(disp), W,
+                                 _stream(stream)))
+    else:
+        _check(lib().tdvh_dp(_ctx(ctx), ct.byref(cd), p_occ, flags, _ptr(disp), W,
+                             _stream(stream)))
     return disp
 
 
@@ -456,7 +469,7 @@ class Pipeline:
 
     def __init__(self, width, height, depth, p1=8, p2=32, flags=PIPE_PREFILTER, ctx=None):
         self.width, self.height, self.depth = width, height, depth
-        params = SgmParams(p1, p2, 0xFF, 0)
+        params = sgm_params(p1, p2)
         h = ct.c_void_p()
         _check(lib().tdvh_pipeline_create(_ctx(ctx), width, height, depth, ct.byref(params),
                                           flags, ct.byref(h)))
