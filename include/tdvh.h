@@ -215,6 +215,21 @@ int tdvh_fast_wta_ncc(tdvh_ctx *ctx, const uint8_t *left,
                       int depth, int radius, int zero_mean, int16_t *disp,
                       int disp_pitch, void *stream);
 
+/* tdvh_fast_wta_ncc without any device allocation and without a context:
+ * the box sums live in a 64 MiB array of device memory that each GPU holds
+ * from the library's load until the process ends, so images up to 4,194,304
+ * pixels (W*H*16 bytes) are accepted; larger ones return
+ * TDVH_ERR_UNSUPPORTED. Runs on the calling thread's current device and
+ * `stream`. Calls on one GPU share the array: calls that overlap on one GPU
+ * (other streams, other threads) race and must be serialised by the caller.
+ * No counterpart in the reference; for callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). */
+int tdvh_fast_wta_ncc_noalloc(const uint8_t *left, const uint8_t *right, int width,
+                              int height, int pitch, int depth, int radius,
+                              int zero_mean, int16_t *disp, int disp_pitch,
+                              void *stream);
+
 typedef enum {
     TDVH_FUSE_STAGED = 0, /* BT volume in HBM, then SGM, then WTA */
     TDVH_FUSE_COST = 1    /* BT recomputed inside the SGM scan; no C volume */

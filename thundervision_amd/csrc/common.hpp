@@ -118,6 +118,9 @@ struct tdvh_ctx {
 namespace tdvh {
 // Makes the context's device current for the calling thread.
 int use_device(tdvh_ctx *ctx);
+// Entry points without a context: the launch hooks record nothing (no
+// profiler events are created).
+void use_no_context();
 } // namespace tdvh
 
 // ---- device helpers ----------------------------------------------------------

@@ -507,14 +507,11 @@ static int check_images(const uint8_t *l, const uint8_t *r, int W, int H, int pi
     return TDVH_OK;
 }
 
-// Per-pixel window sums and reciprocal roots of both images into the
-// context's scratch (box_sums_kernel).
-static int ncc_box_sums(tdvh_ctx *ctx, const uint8_t *left, const uint8_t *right, int width,
-                        int height, int pitch, int radius, int zero_mean, const int32_t **out,
-                        hipStream_t s)
+// Per-pixel window sums and reciprocal roots of both images into `box`
+// (4 planes of W*H words; box_sums_kernel).
+static int launch_box_sums(const uint8_t *left, const uint8_t *right, int width, int height,
+                           int pitch, int radius, int zero_mean, int32_t *box, hipStream_t s)
 {
-    void *box = nullptr;
-    TDVH_TRY(ctx->scratch("ncc_box", (size_t)width * height * 16, &box));
     dim3 grid((width + 255) / 256, height);
     prof_begin(s);
     switch (radius) {
@@ -526,8 +523,52 @@ static int ncc_box_sums(tdvh_ctx *ctx, const uint8_t *left, const uint8_t *right
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5)
 #undef CASE
     }
-    *out = (const int32_t *)box;
     return check_launch("box_sums_kernel", s);
+}
+
+// The same into the context's scratch.
+static int ncc_box_sums(tdvh_ctx *ctx, const uint8_t *left, const uint8_t *right, int width,
+                        int height, int pitch, int radius, int zero_mean, const int32_t **out,
+                        hipStream_t s)
+{
+    void *box = nullptr;
+    TDVH_TRY(ctx->scratch("ncc_box", (size_t)width * height * 16, &box));
+    *out = (const int32_t *)box;
+    return launch_box_sums(left, right, width, height, pitch, radius, zero_mean, (int32_t *)box,
+                           s);
+}
+
+// ---- fused NCC + WTA without allocation --------------------------------------
+//
+// Device memory of tdvh_fast_wta_ncc_noalloc: the NCC box sums (16 bytes per
+// pixel) of one call. Each GPU holds its own copy from the library's load
+// until the process ends; its address on a device comes from
+// hipGetSymbolAddress with that device current (the array's name is not a
+// device address in host code). Calls on one GPU share it.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static void *g_scratch_ptr[NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mutex;
+
+// Device address of the current device's scratch array, looked up once per
+// device.
+static int noalloc_scratch(void **out)
+{
+    int dev = 0;
+    TDVH_CHECK_HIP(hipGetDevice(&dev));
+    TDVH_REQUIRE(dev >= 0 && dev < NOALLOC_MAX_DEVICES, "device %d above %d", dev,
+                 NOALLOC_MAX_DEVICES - 1);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        TDVH_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        TDVH_REQUIRE(p != nullptr, "no device address for the scratch array");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return TDVH_OK;
 }
 
 } // namespace tdvh
@@ -594,5 +635,31 @@ extern "C" int tdvh_fast_wta_ncc(tdvh_ctx *ctx, const uint8_t *left, const uint8
     const int32_t *box = nullptr;
     TDVH_TRY(ncc_box_sums(ctx, left, right, width, height, pitch, radius, zero_mean, &box, s));
     WinArgs a{left, right, width, height, pitch, depth, 0, 0, 0, disp, disp_pitch, box, zero_mean};
+    return launch_window<WIN_NCC_WTA>(a, radius, s);
+}
+
+extern "C" int tdvh_fast_wta_ncc_noalloc(const uint8_t *left, const uint8_t *right, int width,
+                                         int height, int pitch, int depth, int radius,
+                                         int zero_mean, int16_t *disp, int disp_pitch,
+                                         void *stream)
+{
+    clear_error();
+    use_no_context();
+    TDVH_TRY(check_images(left, right, width, height, pitch, depth, WIN_WTA_MAX_DEPTH));
+    TDVH_REQUIRE(disp && disp_pitch >= width, "bad disparity output");
+    TDVH_REQUIRE(radius >= 1 && radius <= 5, "window radius %d outside [1, 5]", radius);
+    const size_t need = (size_t)width * height * 16;
+    if (need > NOALLOC_SCRATCH_BYTES) {
+        set_error("%dx%d needs %zu bytes of box sums, above the %zu-byte scratch array", width,
+                  height, need, NOALLOC_SCRATCH_BYTES);
+        return TDVH_ERR_UNSUPPORTED;
+    }
+    void *box = nullptr;
+    TDVH_TRY(noalloc_scratch(&box));
+    hipStream_t s = (hipStream_t)stream;
+    TDVH_TRY(launch_box_sums(left, right, width, height, pitch, radius, zero_mean, (int32_t *)box,
+                             s));
+    WinArgs a{left, right, width, height, pitch, depth, 0, 0, 0, disp, disp_pitch,
+              (const int32_t *)box, zero_mean};
     return launch_window<WIN_NCC_WTA>(a, radius, s);
 }

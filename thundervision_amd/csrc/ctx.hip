@@ -81,6 +81,8 @@ int use_device(tdvh_ctx *ctx)
     return TDVH_OK;
 }
 
+void use_no_context() { g_cur_ctx = nullptr; }
+
 } // namespace tdvh
 
 int tdvh_ctx::scratch(const char *name, size_t bytes, void **out)

@@ -79,6 +79,7 @@ SIGNATURES = {
     "tdvh_dp_f32": (_I, [_VP, ct.POINTER(DSI), ct.c_float, _I, _VP, _I, _VP]),
     "tdvh_fast_wta_ssd": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, _I, _VP, _I, _VP]),
     "tdvh_fast_wta_ncc": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _VP, _I, _VP]),
+    "tdvh_fast_wta_ncc_noalloc": (_I, [_VP, _VP, _I, _I, _I, _I, _I, _I, _VP, _I, _VP]),
     "tdvh_stereo_bt_sgm": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, ct.POINTER(SgmParams), _I, _VP,
                                 _I, _VP]),
     "tdvh_disp_to_unit_float": (_I, [_VP, _VP, _I, _I, _I, _I, _VP, _I, _VP]),
