@@ -1,0 +1,124 @@
+"""GPU parity at the workloads' full sizes against the CPU oracle: BT + SGM-8
++ WTA at 1920x1080 D=256 and 1280x720 D=192 (fused and staged), the
+reference geometry with every quirk flag at 384x288 D=16, zero-mean NCC
+over the whole 1920x1080 D=256 volume against the FP64 oracle, and the
+fallback line-scan and plane-sum kernels forced on."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from thundervision_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def to_dev(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+@pytest.mark.parametrize("spec", [
+    synth.WINDOW_1080P,                                                  # 1920x1080, D=256, seed 3
+    dict(width=1280, height=720, dmax=191, nrect=12, depth=192, seed=2008),  # a D=192 stream
+], ids=["1080p_d256", "720p_d192"])
+def test_bt_sgm_full_size_against_oracle(tv, dev, spec):
+    L, R, gt = synth.workload_pair(spec)
+    D = spec["depth"]
+    want = oracle.stereo_bt_sgm(L, R, D)
+    Ld, Rd = to_dev(L, dev), to_dev(R, dev)
+    for fusion in (tv.FUSE_COST, tv.FUSE_STAGED):
+        got = tv.stereo_bt_sgm(Ld, Rd, D, fusion=fusion).cpu().numpy()
+        assert (got == want).all(), (fusion, int((got != want).sum()))
+    assert (want == gt).mean() > 0.7
+
+
+@pytest.mark.parametrize("flags", [1, 1 | 2, 1 | 4, 1 | 2 | 4])
+def test_reference_geometry_at_384x288(tv, dev, flags):
+    """The 384x288 D=16 workload in the reference's path geometry (its W != H
+    end points leave 25% of two diagonal directions unvisited and revisit
+    288 pixels up to 97 times), with start-assign and the minimum that skips
+    the last disparity."""
+    L, R, _ = synth.workload_pair(synth.PLAIN_384)
+    C = oracle.cost_bt2(L, R, 16)
+    oflags = (oracle.SGM_START_ASSIGN if flags & 2 else 0) | \
+             (oracle.SGM_MIN_SKIP_LAST if flags & 4 else 0)
+    S_want = oracle.sgm_ref(C, 8, 32, oflags)
+    disp, S = tv.sgm(to_dev(C.view(np.int16), dev), flags=flags, return_S=True)
+    assert (S.cpu().numpy() == S_want).all()
+    assert (disp.cpu().numpy() == oracle.wta(S_want)).all()
+
+
+def test_ncc_full_1080p_against_fp64(tv, dev):
+    """Every cell of the 9x9 zero-mean NCC volume at 1920x1080, D=256 within
+    |cost - cost_fp64| <= 1e-5 * max(|rho|, 1e-2), and the argmin
+    disparities identical except where the two best FP64 costs are within
+    1e-6. The FP64 oracle runs over bands of rows with a 4-row halo (inner
+    rows only compared), which keeps its memory small; the window sums of
+    inner rows never reach a band edge, so they equal the full image's."""
+    L, R, _ = synth.workload_pair(synth.WINDOW_1080P)
+    H, W = L.shape
+    D, r = 256, 4
+    got = tv.cost_ncc(to_dev(L, dev), to_dev(R, dev), D, r, True)
+    disp = tv.wta(got).cpu().numpy()
+    band = 120
+    worst = 0.0
+    mism = 0
+    for y0 in range(0, H, band):
+        y1 = min(H, y0 + band)
+        a, b = max(0, y0 - r), min(H, y1 + r)
+        C64, rho = oracle.cost_ncc(L[a:b], R[a:b], D, r, True)
+        C64, rho = C64[y0 - a:y1 - a], rho[y0 - a:y1 - a]
+        g = got[y0:y1].cpu().numpy()
+        fin = np.isfinite(C64)
+        assert (np.isfinite(g) == fin).all(), y0
+        err = np.abs(g[fin].astype(np.float64) - C64[fin])
+        bound = 1e-5 * np.maximum(np.abs(rho[fin]), 1e-2)
+        worst = max(worst, float((err / bound).max()))
+        want = oracle.wta(C64)
+        diff = disp[y0:y1] != want
+        if diff.any():
+            best2 = np.sort(C64[diff], axis=1)[:, :2]  # +inf sorts last
+            assert ((best2[:, 1] - best2[:, 0]) < 1e-6).all(), y0
+            mism += int(diff.sum())
+    assert worst <= 1.0, worst
+    assert mism < 1000
+
+
+_FORCED = r"""
+import sys
+import numpy as np, torch
+sys.path.insert(0, %r)
+import oracle
+import thundervision_amd.tdvh as tv
+from thundervision_amd import synth
+dev = torch.device("cuda:0")
+for (W, H, D, p2) in [(96, 40, 16, 32), (130, 57, 64, 32), (70, 33, 100, 32), (64, 48, 256, 32),
+                      (80, 30, 48, 300)]:
+    L, R, _ = synth.pair(W, H, D - 1, 8, W + D)
+    want = oracle.wta(oracle.sgm_std(oracle.cost_bt2(L, R, D), 8, p2))
+    Ld, Rd = torch.from_numpy(L).to(dev), torch.from_numpy(R).to(dev)
+    for fusion in (tv.FUSE_COST, tv.FUSE_STAGED):
+        got = tv.stereo_bt_sgm(Ld, Rd, D, p2=p2, fusion=fusion).cpu().numpy()
+        assert (got == want).all(), (W, H, D, p2, fusion)
+print("forced ok")
+"""
+
+
+@pytest.mark.parametrize("env", [{"TDVH_SGM_LINES": "flat"}, {"TDVH_SGM_SUM": "narrow"},
+                                 {"TDVH_SGM_LINES": "flat", "TDVH_SGM_SUM": "narrow"}],
+                         ids=["flat_lines", "narrow_sum", "both"])
+def test_forced_fallback_kernels(env):
+    """The flat-address line kernel (taken for planes of 2 GiB and more) and
+    the one-row-per-pixel plane sum, forced on in a child process (the
+    library reads these switches once per process)."""
+    e = dict(os.environ)
+    e.update(env)
+    r = subprocess.run([sys.executable, "-c", _FORCED % ROOT], env=e, capture_output=True,
+                       text=True, timeout=100)
+    assert r.returncode == 0 and "forced ok" in r.stdout, r.stdout + r.stderr

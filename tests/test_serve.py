@@ -154,3 +154,39 @@ def test_frames_identical_alone_and_in_the_mix(tv, dev):
         assert torch.equal(alone, disp), (f.stream, f.index)
     st = serve.latency_stats(done)
     assert st["p99_ms"] >= st["p50_ms"] > 0
+
+
+@pytest.mark.gpu
+def test_cfg5_served_mix_frames_identical_alone(tv, dev):
+    """The served workload: 16 streams (4 each at 640x480/D64, 1280x720/D128,
+    1280x720/D192, 1920x1080/D256), Poisson arrivals at 600 pairs/s in total,
+    a fresh frame per arrival (rectangles drifting with the frame index),
+    64 GB budget, 4 slots with the benchmark's cell budget and backfill.
+    Every served disparity map equals the same frame computed alone."""
+    import torch
+    from thundervision_amd import synth
+    shapes = serve.SERVED_STREAMS
+    sched = serve.poisson_schedule(len(shapes), 600.0, 0.15)
+    frames = []
+    for t, s, k in sched:
+        W, H, D = shapes[s]
+        L, R, _ = synth.pair(W, H, D - 1, 12, 2000 + s, frame=k)
+        Ld, Rd = torch.from_numpy(L).to(dev), torch.from_numpy(R).to(dev)
+        disp = torch.full((H, W), -1, dtype=torch.int16, device=dev)
+        frames.append(serve.Frame(t, s, k, W, H, D, (Ld, Rd, disp)))
+    big = max(W * H * D for W, H, D in shapes)
+    ex = serve.GpuExecutor(slots=4)
+    sch = serve.Scheduler(ex, 4, 64 << 30, cell_budget=1.3 * big, backfill_s=0.003)
+    ex.warm(frames)
+    for f in frames:
+        f.payload[2].fill_(-1)
+    ex.start()
+    done = sch.run(frames)
+    torch.cuda.synchronize()
+    assert len(done) == len(frames) >= 40
+    assert {(f.width, f.height, f.depth) for f in frames} == set(shapes)
+    assert sch.max_concurrent >= 2 and sch.peak_in_use <= 64 << 30
+    for f in frames:
+        Ld, Rd, disp = f.payload
+        alone = tv.stereo_bt_sgm(Ld, Rd, f.depth)
+        assert torch.equal(alone, disp), (f.stream, f.index)
