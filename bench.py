@@ -51,6 +51,21 @@ def measured_traffic(kernel):
         return None
 
 
+def step_ms_hip_events(step, stream, torch, steps):
+    """Mean milliseconds of one step between two hipEvents recorded on the
+    stream the kernels are launched on, over `steps` back-to-back steps."""
+    step()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -111,57 +126,91 @@ def bench_batched(tv, synth, torch, ctx, reps, specs=None, workload=MIX_WORKLOAD
             "plain_per_pair_ms": round(ms_plain, 3),
             "plain_per_pair_value": round(cells / (ms_plain * 1e-3) / 1e6, 1),
             "speedup_vs_plain": round(ms_plain / ms_graph, 3),
-            "tuned": {"waves_per_group": tuned["waves_per_group"], "prefetch": tuned["prefetch"],
-                      "cached": tuned["cached"]},
+            "tuned": {k: tuned[k] for k in ("fusion_depth", "lines_per_wave", "waves_per_group",
+                                              "prefetch", "cached")},
+            "tune_trials_ms": [dict(t, ms=round(t["ms"], 3)) for t in tuned["trials"]],
             "bit_exact_vs_plain": exact}
 
 
-def bench_served(torch, duration=1.0, rate=600.0, frames_per_stream=4, slots=4):
-    """16 streams (4 each at 640x480/D64, 1280x720/D128, 1280x720/D192,
-    1920x1080/D256), Poisson arrivals at `rate` pairs/s in total, frames of
-    stream s from the generator with seed 2000+s and drifting rectangles.
-    Continuous batching over `slots` concurrent slots under a 64 GB budget,
-    with at most 1.3 largest frames' cells in flight and 3 ms of backfill past
-    a blocked head frame, against one slot (the plain per-pair call) at the
-    same load."""
+SERVED_POLICIES = {
+    # name: (slots, Scheduler keyword arguments; cell budgets in largest frames)
+    "plain_per_pair": (1, {}),
+    "served": (4, {"cell_budget": 1.3, "backfill_s": 0.003}),
+}
+
+
+def served_frames(torch, dev, duration, rate, distinct):
+    """The served workload's arrivals (Poisson, seeds 2000+stream) over
+    `duration` seconds and the distinct frames they use: frame k of stream s
+    is the generator's frame k % distinct of that stream (rectangles drifting
+    1-3 px per frame), generated on a thread pool and kept on the device."""
+    from concurrent.futures import ThreadPoolExecutor
+
     from thundervision_amd import serve, synth
+    shapes = serve.SERVED_STREAMS
+    keys = [(s, k) for s in range(len(shapes)) for k in range(distinct)]
+
+    def gen(key):
+        s, k = key
+        W, H, D = shapes[s]
+        L, R, _ = synth.pair(W, H, D - 1, 12, 2000 + s, frame=k)
+        return key, L, R
+
+    pool = {}
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        for key, L, R in ex.map(gen, keys):
+            pool[key] = (torch.from_numpy(L).to(dev), torch.from_numpy(R).to(dev))
+    return serve.poisson_schedule(len(shapes), rate, duration), pool
+
+
+def bench_served(torch, tv, duration=5.0, rate=600.0, distinct=24, policies=None):
+    """16 streams (4 each at 640x480/D64, 1280x720/D128, 1280x720/D192,
+    1920x1080/D256), Poisson arrivals at `rate` pairs/s in total, each stream
+    cycling through `distinct` generator frames. Every policy serves the same
+    arrivals: one slot (the plain per-pair call back to back) and continuous
+    batching over concurrent slots under a 64 GB budget. Every served
+    disparity map is checked against the same frame computed alone; p50/p99
+    come from device completion events."""
+    from thundervision_amd import serve
     dev = torch.device("cuda:0")
     shapes = serve.SERVED_STREAMS
-    inputs = {}
-    for s, (W, H, D) in enumerate(shapes):
-        for k in range(frames_per_stream):
-            L, R, _ = synth.pair(W, H, D - 1, 12, 2000 + s, frame=k)
-            inputs[(s, k)] = (torch.from_numpy(L).to(dev), torch.from_numpy(R).to(dev))
-    sched = serve.poisson_schedule(len(shapes), rate, duration)
+    sched, pool = served_frames(torch, dev, duration, rate, distinct)
+    alone = {key: tv.stereo_bt_sgm(L, R, shapes[key[0]][2]) for key, (L, R) in pool.items()}
+    torch.cuda.synchronize()
+    big = max(W * H * D for W, H, D in shapes)
     out = {"workload": "16 streams: 4x 640x480/D64, 4x 1280x720/D128, 4x 1280x720/D192, "
-                       "4x 1920x1080/D256; Poisson %.0f pairs/s for %.1f s; BT+SGM8+WTA"
-                       % (rate, duration),
-           "frames": len(sched)}
-    for name, nslots in (("served", slots), ("plain_per_pair", 1)):
+                       "4x 1920x1080/D256; Poisson %.0f pairs/s for %.1f s; %d distinct "
+                       "drifting frames per stream; BT+SGM8+WTA" % (rate, duration, distinct),
+           "frames": len(sched), "all_frames_exact": True}
+    for name, (nslots, kw) in (policies or SERVED_POLICIES).items():
         frames = []
         for t, s, k in sched:
             W, H, D = shapes[s]
-            L, R = inputs[(s, k % frames_per_stream)]
-            disp = torch.empty((H, W), dtype=torch.int16, device=dev)
+            L, R = pool[(s, k % distinct)]
+            disp = torch.full((H, W), -1, dtype=torch.int16, device=dev)
             frames.append(serve.Frame(t, s, k, W, H, D, (L, R, disp)))
         ex = serve.GpuExecutor(nslots)
-        big = max(W * H * D for W, H, D in shapes)
-        if nslots > 1:
-            sch = serve.Scheduler(ex, nslots, 64 << 30, cell_budget=1.3 * big, backfill_s=0.003)
-        else:
-            sch = serve.Scheduler(ex, nslots, 64 << 30)
+        kw = dict(kw)
+        if "cell_budget" in kw:
+            kw["cell_budget"] = kw["cell_budget"] * big
+        sch = serve.Scheduler(ex, nslots, 64 << 30, **kw)
         # every slot runs every frame shape once before the clock starts
         ex.warm(frames)
         ex.start()
         done = sch.run(frames)
         torch.cuda.synchronize()
+        exact = len(done) == len(frames) and all(
+            torch.equal(f.payload[2], alone[(f.stream, f.index % distinct)]) for f in frames)
+        out["all_frames_exact"] = out["all_frames_exact"] and exact
         st = serve.latency_stats(done)
-        out[name] = {k: round(v, 3) for k, v in st.items()}
-        out[name]["slots"] = nslots
-        out[name]["cell_budget"] = sch.cell_budget
-        out[name]["backfill_ms"] = None if sch.backfill_s is None else sch.backfill_s * 1e3
-        out[name]["max_concurrent"] = sch.max_concurrent
-        out[name]["peak_inflight_GB"] = round(sch.peak_in_use / 2**30, 2)
+        row = {k: round(v, 3) for k, v in st.items()}
+        row.update(slots=nslots, exact=exact, max_concurrent=sch.max_concurrent,
+                   peak_inflight_GB=round(sch.peak_in_use / 2**30, 2),
+                   policy={k: (round(v, 4) if isinstance(v, float) else v) for k, v in kw.items()},
+                   by_shape={k: {kk: round(vv, 3) for kk, vv in v.items()}
+                             for k, v in serve.latency_by_shape(done).items()})
+        out[name] = row
+        del frames, done
     out["offered_MDE/s"] = round(sum(W * H * D for t, s, k in sched
                                      for (W, H, D) in [shapes[s]]) / duration / 1e6, 1)
     return out
@@ -248,7 +297,7 @@ def bench_plain_384(tv, synth, torch, ctx, with_cpu, reps=50):
         row["cpu_ms"] = {"bt": round((t1 - t0) * 1e3, 2), "sgm": round((t2 - t1) * 1e3, 2),
                          "wta": round((t3 - t2) * 1e3, 2)}
         row["cpu_MDE/s"] = round(cells / (t3 - t0) / 1e6, 2)
-        row["cpu"] = "oracle/tdvref.c gcc -O3, 1 thread (%s, nproc %d)" % (cpu_model(),
+        row["cpu"] = "oracle/tdvref.c gcc -O3 -march=x86-64-v3, 1 thread (%s, nproc %d)" % (cpu_model(),
                                                                            os.cpu_count() or 0)
         row["bit_exact"] = bool((out.cpu().numpy() == d).all())
     return row
@@ -413,7 +462,9 @@ def main():
     other = tv.FUSE_STAGED if fusion == tv.FUSE_COST else tv.FUSE_COST
     sec_other = timed(other, max(5, args.steps // 5), 2)
 
-    # per-kernel hipEvent timing on the launch stream
+    # the step on the launch stream's hipEvents, and per-kernel hipEvent
+    # timing (the library's profiler, same stream)
+    step_ms = step_ms_hip_events(step, stream, torch, args.steps)
     ctx.profile(True)
     ctx.profile_reset()
     for _ in range(max(5, args.steps // 5)):
@@ -423,8 +474,18 @@ def main():
     ctx.profile(False)
     dom = max(table, key=lambda k: table[k][0])
     dom_ms = table[dom][0] / table[dom][1]
-    achieved = b_alg(W, H, D) / (dom_ms * 1e-3) / 1e9
-    pipeline_ms = sum(v[0] / v[1] for v in table.values())
+    # roofline of the whole step (three kernels: BT term planes, line scans,
+    # plane sum + WTA): the algorithmic bytes of the staged uint16 model over
+    # the step's hipEvent time; the measured HBM bytes of the three kernels
+    # (PMC) beside them
+    achieved = b_alg(W, H, D) / (step_ms * 1e-3) / 1e9
+    per_kernel = {}
+    traffic = 0
+    for k, (tot, cnt) in table.items():
+        tb = measured_traffic(k) if args.mode == "fused" else None
+        per_kernel[k] = {"avg_ms": round(tot / cnt, 4), "measured_bytes": tb,
+                         "measured_GB/s": round(tb / (tot / cnt * 1e-3) / 1e9, 1) if tb else None}
+        traffic = traffic + tb if (tb is not None and traffic is not None) else None
 
     batched = batched_hd = None
     if not args.no_batched:
@@ -434,7 +495,7 @@ def main():
 
     served = None
     if not args.no_served:
-        served = bench_served(torch)
+        served = bench_served(torch, tv)
 
     boundary = bench_boundary(tv, torch, ctx, L, R, D)
     window = bench_window(tv, synth, torch, ctx)
@@ -451,11 +512,18 @@ def main():
             want = oracle.stereo_bt_sgm(L, R, D)
         cs = (time.perf_counter() - t0) / reps
         cpu = {"value": round(cells / cs / 1e6, 3), "unit": "MDE/s", "cores": 1, "kind": "port",
-               "sample": "%d full 1280x720 D=128 pairs, BT+SGM8+WTA, oracle/tdvref.c gcc -O3 "
+               "sample": "%d full 1280x720 D=128 pairs, BT+SGM8+WTA, oracle/tdvref.c gcc -O3 -march=x86-64-v3 "
                          "single thread (%s)" % (reps, cpu_model())}
         got = disp.cpu().numpy()
         if not (got == want).all():
             print("ERROR: GPU disparities differ from the oracle", file=sys.stderr)
+            sys.exit(1)
+    if served is not None and not served["all_frames_exact"]:
+        print("ERROR: a served frame differs from the same frame computed alone", file=sys.stderr)
+        sys.exit(1)
+    for bt in (batched, batched_hd):
+        if bt is not None and not bt["bit_exact_vs_plain"]:
+            print("ERROR: a batched pair differs from the plain per-pair call", file=sys.stderr)
             sys.exit(1)
 
     out = {
@@ -477,10 +545,13 @@ def main():
                    "path": args.mode, "width": W, "height": H, "depth": D},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(dom) if args.mode == "fused" else None,
-                     "kernel": dom,
-                     "kernel_avg_ms": round(dom_ms, 4),
-                     "alg_bytes_per_launch": b_alg(W, H, D)},
+                     "traffic": traffic,
+                     "scope": "whole step: bt_planes_kernel + sgm_lines_kernel + "
+                              "sgm_sum_wta_kernel (one pair)",
+                     "alg_bytes_per_step": b_alg(W, H, D),
+                     "step_ms_hip_events": round(step_ms, 4),
+                     "dominant_kernel": dom, "dominant_kernel_avg_ms": round(dom_ms, 4),
+                     "kernels": per_kernel},
         "pipeline_frac_of_hbm_roofline": round(b_alg(W, H, D) / (sec * 1e9) / HBM_PEAK_GBS, 4),
         "other_path": {"path": "staged" if other == tv.FUSE_STAGED else "fused",
                        "value": round(cells / sec_other / 1e6, 1),

@@ -268,10 +268,13 @@ size_t tdvh_sgm_paths(int width, int height, tdvh_sgpath *paths);
 
 /* ---- batched execution ------------------------------------------------- */
 /* Many pairs of uneven size and disparity range planned once into device
- * work-item lists (BT rows, 4-line SGM groups ordered longest first, pixel
- * chunks) and run as one captured hipGraph whose depth classes are parallel
- * branches. There is no counterpart in the reference; it runs the same
- * arithmetic as tdvh_stereo_bt_sgm on every pair. */
+ * work-item lists (BT rows, line groups of one SGM direction ordered longest
+ * first, runs of pixels for the plane sums) and run as one captured
+ * hipGraph whose depth classes are parallel branches, each a persistent
+ * launch fed by a device work queue or one launch per phase
+ * (tdvh_batch_config). There is no counterpart in the reference (it runs
+ * one launch per pair, semiglobal.cu:507-516); every pair gets the
+ * arithmetic of tdvh_stereo_bt_sgm. */
 typedef struct {
     const uint8_t *left, *right; /* device */
     int32_t width, height, pitch, depth;
@@ -362,21 +365,33 @@ int tdvh_pipeline_destroy(tdvh_pipeline *p);
 
 typedef struct tdvh_batch tdvh_batch;
 
-/* Plans a batch (work queue, scratch) for `n` pairs. The pair descriptors
+/* Plans a batch for `n` pairs (item lists, scratch). The pair descriptors
  * are copied; their device buffers must stay valid while the batch runs. */
 int tdvh_batch_create(tdvh_ctx *ctx, const tdvh_pair *pairs, int n,
                       const tdvh_sgm_params *params, tdvh_batch **out);
 int tdvh_batch_destroy(tdvh_batch *b);
-/* Tile configuration (autotuner knobs): lines per wave (4), waves per
- * workgroup (1, 2, 4; default 1) and prefetch depth of the SGM line scans
- * (4 or 8 steps; default 8, at most 4 for D > 128); 0 keeps the current
- * value. */
+/* Execution knobs (autotuner), 0 keeps the current value:
+ *  lines_per_wave (tile height of the SGM line scans): 4 (16 lanes per
+ *    line) or 8 (8 lanes per line, used for pairs with D <= 64);
+ *  waves_per_group: 1, 2 or 4 waves per workgroup;
+ *  prefetch: 4 or 8 steps of the line scans' load ring (8 means the
+ *    deepest ring that keeps two waves per SIMD);
+ *  fusion_depth: 1 = one launch per phase (BT, line scans, plane sums);
+ *    2 = BT, then per depth class one persistent launch that takes line
+ *    groups and plane sums from a device work queue, each pair's sums
+ *    released by a completion counter once its 8 directions are done;
+ *    3 = per class one persistent launch for all three phases;
+ *  sum_lag (fusion depth 2 and 3): a pair's plane sums are queued after the
+ *    line groups of the next sum_lag pairs of its class, so that they
+ *    overlap later pairs' scans; -1 queues them after all line groups.
+ * Defaults: 4, 1, 8, 1, -1. */
 int tdvh_batch_config(tdvh_batch *b, int lines_per_wave, int waves_per_group,
-                      int prefetch);
+                      int prefetch, int fusion_depth, int sum_lag);
 /* Enqueues the whole batch on `stream`. With use_graph != 0 the launch
  * sequence is captured once into a hipGraph and replayed. */
 int tdvh_batch_run(tdvh_batch *b, int use_graph, void *stream);
-/* Synchronises `stream` (the batch's results are then complete). */
+/* Synchronises `stream` (the batch's results are then complete); reports a
+ * work queue that stalled. */
 int tdvh_batch_wait(tdvh_batch *b, void *stream);
 /* Total W*H*D cells of the batch. */
 int64_t tdvh_batch_cells(const tdvh_batch *b);

@@ -498,7 +498,8 @@ static void ref_traverse(const uint16_t *C, int W, int H, int D, int P1,
             if (flags & TDVREF_SGM_START_ASSIGN)
                 *s = c;
             else if (*s != TDVREF_INF_I32)
-                *s = (c == TDVREF_INF_I32) ? TDVREF_INF_I32 : *s + c;
+                *s = (c == TDVREF_INF_I32) ? TDVREF_INF_I32
+                                           : (int32_t)((uint32_t)*s + (uint32_t)c);
             last[d] = c;
         }
     }
@@ -526,9 +527,13 @@ static void ref_traverse(const uint16_t *C, int W, int H, int D, int P1,
             if (d + 1 < D && last[d + 1] != TDVREF_INF_I32 &&
                 (int64_t)last[d + 1] + P1 < best)
                 best = (int64_t)last[d + 1] + P1;
-            const int32_t v = (int32_t)((int64_t)cu + best - m);
+            /* path costs and sums are int32 with two's-complement wrap (the
+             * device's atomic adds wrap; values only leave the int32 range
+             * when the minimum skipping the last disparity covers no valid
+             * cell) */
+            const int32_t v = (int32_t)(uint32_t)((int64_t)cu + best - m);
             next[d] = v;
-            if (*s != TDVREF_INF_I32) *s += v;
+            if (*s != TDVREF_INF_I32) *s = (int32_t)((uint32_t)*s + (uint32_t)v);
         }
         memcpy(last, next, sizeof(int32_t) * (size_t)D);
     }

@@ -1,9 +1,13 @@
-"""Autotuner of the persistent batch launch.
+"""Autotuner of the batched launch (tdvh_batch_config).
 
-Sweeps the batch knobs (waves per workgroup, prefetch depth of the SGM line
-scans) on the batch itself, keeps the fastest and caches it per batch
-signature (the multiset of pair shapes) in a JSON file, so a service tunes a
-given mix of resolutions once.
+Sweeps the batch knobs on the batch itself, one knob at a time from the
+current best (coordinate descent): fusion depth (one launch per phase, or a
+persistent work-queue launch per depth class for line scans and sums, or
+for all three phases), tile height (lines per wave: 4, or 8 with 8-lane
+lines), waves per workgroup, the prefetch depth of the SGM line scans and,
+for the persistent queues, how far a pair's sums trail its scans.
+The fastest setting is cached per batch signature (the multiset of pair
+shapes) in a JSON file, so a service tunes a given mix of resolutions once.
 """
 import hashlib
 import json
@@ -11,8 +15,12 @@ import os
 
 import torch
 
-WAVES_PER_GROUP = (1, 2, 4)
-PREFETCH = (4, 8)
+# knob name -> candidate values, in sweep order
+KNOBS = (("fusion_depth", (1, 2, 3)), ("lines_per_wave", (4, 8)),
+         ("waves_per_group", (1, 2, 4)), ("prefetch", (4, 8)), ("sum_lag", (-1, 1, 4)))
+DEFAULT = {"fusion_depth": 1, "lines_per_wave": 4, "waves_per_group": 1, "prefetch": 8,
+           "sum_lag": -1}
+WAVES_PER_GROUP = dict(KNOBS)["waves_per_group"]
 
 
 def cache_path():
@@ -57,24 +65,44 @@ def time_batch(batch, reps=3, use_graph=False):
     return start.elapsed_time(end) / reps
 
 
-def tune(batch, shapes, reps=3, path=None, force=False):
+def _apply(batch, knobs):
+    batch.config(knobs["lines_per_wave"], knobs["waves_per_group"], knobs["prefetch"],
+                 knobs["fusion_depth"], knobs.get("sum_lag", -1))
+
+
+def tune(batch, shapes, reps=3, path=None, force=False, use_graph=True):
     """Configures `batch` with the cached or freshly measured best knobs.
-    Returns {"waves_per_group", "prefetch", "ms", "cached", "trials"}."""
+    Returns the knobs plus {"ms", "cached", "trials"}."""
     path = path or cache_path()
     key = signature(shapes)
     data = _load(path)
     if key in data and not force:
         best = data[key]
-        batch.config(0, best["waves_per_group"], best["prefetch"])
+        _apply(batch, best)
         return dict(best, cached=True, trials=[])
+    cur = dict(DEFAULT)
+    seen = {}
     trials = []
-    for wpg in WAVES_PER_GROUP:
-        for pf in PREFETCH:
-            batch.config(0, wpg, pf)
-            ms = time_batch(batch, reps)
-            trials.append({"waves_per_group": wpg, "prefetch": pf, "ms": ms})
-    best = min(trials, key=lambda t: t["ms"])
-    batch.config(0, best["waves_per_group"], best["prefetch"])
+
+    def measure(knobs):
+        k = tuple(sorted(knobs.items()))
+        if k not in seen:
+            _apply(batch, knobs)
+            seen[k] = time_batch(batch, reps, use_graph=use_graph)
+            trials.append(dict(knobs, ms=seen[k]))
+        return seen[k]
+
+    best_ms = measure(cur)
+    for name, values in KNOBS:
+        if name == "sum_lag" and cur["fusion_depth"] == 1:
+            continue  # only the persistent queues order sums
+        for v in values:
+            cand = dict(cur, **{name: v})
+            ms = measure(cand)
+            if ms < best_ms:
+                best_ms, cur = ms, cand
+    _apply(batch, cur)
+    best = dict(cur, ms=best_ms)
     data[key] = best
     _store(path, data)
     return dict(best, cached=False, trials=trials)

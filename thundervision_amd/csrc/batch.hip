@@ -1,19 +1,34 @@
 // Batched execution: many stereo pairs of uneven size and disparity range
-// run as one hipGraph.
+// (BT + SGM-8 + WTA each), captured as one hipGraph.
 //
-// The batch is planned once into device work-item lists (one wave per item):
-//   BT term rows of every pair                     (one launch for all pairs)
-//   4-line groups of one SGM direction of a pair   (one launch per depth class)
-//   chunks of pixels: plane sum + WTA              (one launch per depth class)
-// Pairs are grouped by m = ceil(D/32), the register shape of the SGM code, so
-// each class's kernel is compiled for its own m and keeps its occupancy; the
-// line items of a class are ordered longest line first across all its pairs,
-// so the longest recurrences start first. Kernel boundaries order the phases;
-// the depth classes run as parallel branches of the graph. The arithmetic is
-// the per-pair path's (the same device functions), so every pair's
-// disparities equal the plain call's.
+// Work items, one wave each:
+//   BT    rows of a pair's term planes
+//   LINES a group of 4 (or 8) consecutive lines of one SGM direction
+//   SUM   a run of pixels of a pair: plane sum + lowest-index WTA
+// Pairs are grouped by m (words per lane, scan_m(D)), the register shape
+// of the SGM code, so each class's kernels are compiled for their own m and
+// keep their occupancy; the classes run as parallel branches of the graph.
+//
+// Fusion depth (how many of the three phases share one launch per class):
+//   1  one launch per phase (BT of all pairs; per class the line scans, then
+//      the sums), kernel boundaries ordering the phases;
+//   2  BT of all pairs, then per class one persistent launch that runs the
+//      line groups and the sums from a device work queue;
+//   3  per class one persistent launch running BT, line groups and sums.
+// Persistent launches: each wave takes the next item of its class's queue
+// with one global atomic add; items are queued so that an item's
+// dependencies come earlier (BT of every pair, then the line groups pair by
+// pair, longest first within a pair, with a pair's sums queued after the
+// line groups of the next `sum_lag` pairs, or after all of them). A LINES item waits for its pair's BT counter, a
+// SUM item for its pair's LINES counter; every wait is on items already
+// taken by running waves, so it ends (a spin limit bounds it anyway and sets
+// an error flag that tdvh_batch_wait reports). A wave publishes finished
+// items of one pair and kind together (one device-scope release and one
+// atomic add when it moves to another pair or kind, or before it waits) and
+// acquires once per pair it waits on. The arithmetic is the per-pair path's
+// (the same device functions), so every pair's disparities equal the plain
+// call's.
 #include <algorithm>
-#include <cstdlib>
 #include <vector>
 
 #include "sgm_device.hpp"
@@ -27,24 +42,54 @@ struct BPair {
     int16_t *disp;
 };
 
+enum { IT_BT = 0, IT_LINES = 1, IT_SUM = 3 };
+
 struct BItem {
-    int pair, kind, a, b;
+    int pair, kind, a, b; // BT: first row, rows; LINES: direction, first line;
+                          // SUM: first wave unit, wave units
 };
 
-constexpr int SUM_CHUNK = 64; // pixels per sum item
+constexpr int BT_ROWS = 4;             // rows per BT item
+constexpr int SUM_UNITS_PERSIST = 64;  // wave units per queued SUM item
+constexpr uint32_t SPIN_LIMIT = 1u << 24;
+// control words: a queue head per class, the error flag, then per pair the
+// BT and LINES completion counters
+constexpr int CTL_ERR = 9, CTL_PAIRS = 16;
 
-__global__ __launch_bounds__(256) void batch_bt_kernel(const BPair *__restrict__ pairs,
-                                                       const BItem *__restrict__ items, int n)
+// ---- per-item work (shared by the phase kernels and the persistent one) ----
+
+__device__ __forceinline__ void run_bt_item(const BPair &P, const BItem &it)
 {
-    const int w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (w >= n) return;
-    const BItem it = items[w];
-    const BPair &P = pairs[it.pair];
-    const int Wr = P.a.Wr;
+    const int lane = threadIdx.x & 63, Wr = P.a.Wr;
     for (int y = it.a; y < it.a + it.b; y++)
         for (int i = lane; i < Wr; i += 64)
             bt_planes_at(P.L, P.R, P.a.W, P.a.H, P.pitch, P.m, (uint32_t *)P.a.LT,
                          (uint32_t *)P.a.RT, Wr, i, y);
+}
+
+// Wide plane sum of wave units [u0, u0 + n): unit u covers pixels
+// u * (64 / G) .. + 64 / G - 1, G lanes each.
+template <int M>
+__device__ __forceinline__ void run_sum_units(const BPair &P, int u0, int n)
+{
+    constexpr int G = sum_group<M>();
+    const int lane = threadIdx.x & 63;
+    const long npix = (long)P.a.W * P.a.H;
+    for (int u = u0; u < u0 + n; u++) {
+        const long p = (long)u * (64 / G) + lane / G;
+        if (p < npix) sgm_sum_pixel_wide<M, true>(P.a, 8, nullptr, P.disp, P.a.W, p, lane % G);
+    }
+}
+
+// ---- fusion depth 1: one launch per phase ------------------------------------
+
+__global__ __launch_bounds__(256) void batch_bt_kernel(const BPair *__restrict__ pairs,
+                                                       const BItem *__restrict__ items, int n)
+{
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= n) return;
+    const BItem it = items[w];
+    run_bt_item(pairs[it.pair], it);
 }
 
 template <int M, int PF>
@@ -58,9 +103,9 @@ __global__ __launch_bounds__(256) void batch_lines_kernel(const BPair *__restric
     sgm_line_group<M, true, true, PF>(pairs[it.pair].a, it.a, it.b, it.a, prio);
 }
 
-// Buffer-addressed line groups of one depth class (all 8 planes per pair);
+// Buffer-addressed line groups of one depth class (LPL lanes per line);
 // waves_per_group waves per workgroup, two waves per SIMD.
-template <int M, bool PAD, int PF>
+template <int M, bool PAD, int PF, int LPL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void
 batch_lines_buf_kernel(const BPair *__restrict__ pairs, const BItem *__restrict__ items, int n,
                        int prio)
@@ -69,19 +114,17 @@ batch_lines_buf_kernel(const BPair *__restrict__ pairs, const BItem *__restrict_
                                                  (threadIdx.x >> 6));
     if (w >= n) return;
     const BItem it = items[w];
-    sgm_line_group_buf<M, true, PAD, PF>(pairs[it.pair].a, it.a, it.b, it.a, prio);
+    sgm_line_group_buf<M, true, PAD, PF, LPL>(pairs[it.pair].a, it.a, it.b, it.a, prio);
 }
 
-// Plane sum + WTA with 16-byte lanes (sum_group lanes per pixel, one
-// pass), for word counts 1, 2, 4, 6 and 8. items: one per pair of the
-// class, {pair, 3, first wave, waves}, in increasing first wave; a wave finds
-// its pair by binary search (wave-uniform, scalar loads).
+// Plane sum + WTA: items {pair, SUM, first wave, waves}, one per pair of the
+// class in increasing first wave; a wave finds its pair by binary search
+// (wave-uniform, scalar loads).
 template <int M>
 __global__ __launch_bounds__(256) void batch_sum_wide_kernel(const BPair *__restrict__ pairs,
                                                              const BItem *__restrict__ items,
                                                              int n, int waves)
 {
-    constexpr int G = sum_group<M>();
     const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (w >= waves) return;
     int lo = 0, hi = n - 1;
@@ -91,27 +134,85 @@ __global__ __launch_bounds__(256) void batch_sum_wide_kernel(const BPair *__rest
         else hi = mid - 1;
     }
     const BItem it = items[lo];
-    const BPair &P = pairs[it.pair];
-    const int lane = threadIdx.x & 63;
-    const long p = (long)(w - it.a) * (64 / G) + lane / G;
-    if (p < (long)P.a.W * P.a.H)
-        sgm_sum_pixel_wide<M, true>(P.a, 8, nullptr, P.disp, P.a.W, p, lane % G);
+    run_sum_units<M>(pairs[it.pair], w - it.a, 1);
 }
 
-template <int M>
-__global__ __launch_bounds__(256) void batch_sum_kernel(const BPair *__restrict__ pairs,
-                                                        const BItem *__restrict__ items, int n)
+// ---- fusion depth 2 and 3: one persistent launch per class ----------------------
+
+template <int M, bool PAD, int PF, int LPL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void
+batch_queue_kernel(const BPair *__restrict__ pairs, const BItem *__restrict__ items, int nitems,
+                   const int *__restrict__ need, uint32_t *ctl, int npairs, int mc, int wait_bt,
+                   int prio)
 {
-    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (w >= n) return;
-    const BItem it = items[w];
-    const BPair &P = pairs[it.pair];
-    const int row = (threadIdx.x & 63) >> 4;
-    const long npix = (long)P.a.W * P.a.H;
-    for (int q = row; q < it.b; q += 4) {
-        const long p = (long)it.a + q;
-        if (p < npix) sgm_sum_pixel<M, true, true>(P.a, 8, nullptr, P.disp, P.a.W, p);
+    const int lane = threadIdx.x & 63;
+    uint32_t *head = ctl + mc, *err = ctl + CTL_ERR;
+    uint32_t *bt_done = ctl + CTL_PAIRS, *lines_done = bt_done + npairs;
+    int cur_pair = -1, cur_kind = -1;
+    uint32_t pending = 0;
+    int acq_bt = -1, acq_lines = -1; // pairs whose counters this wave has acquired
+    // publishes this wave's finished items of (cur_pair, cur_kind)
+    auto flush = [&]() {
+        if (pending) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            uint32_t *c = (cur_kind == IT_BT ? bt_done : lines_done) + cur_pair;
+            if (lane == 0) __hip_atomic_fetch_add(c, pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            pending = 0;
+        }
+    };
+    auto wait_for = [&](const uint32_t *c, uint32_t want) -> bool {
+        uint32_t spins = 0;
+        while ((uint32_t)__builtin_amdgcn_readfirstlane(
+                   (int)__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < want) {
+            if (++spins > SPIN_LIMIT) return false;
+            __builtin_amdgcn_s_sleep(8);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        return true;
+    };
+    for (;;) {
+        uint32_t idx = 0;
+        if (lane == 0) idx = __hip_atomic_fetch_add(head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        idx = (uint32_t)__builtin_amdgcn_readfirstlane((int)idx);
+        if (idx >= (uint32_t)nitems) break;
+        const BItem it = items[idx];
+        if (pending && (it.pair != cur_pair || it.kind != cur_kind)) flush();
+        bool ok = true;
+        if (it.kind == IT_LINES && wait_bt && acq_bt != it.pair) {
+            flush();
+            ok = wait_for(bt_done + it.pair, (uint32_t)need[it.pair]);
+            acq_bt = it.pair;
+        } else if (it.kind == IT_SUM && acq_lines != it.pair) {
+            flush();
+            ok = wait_for(lines_done + it.pair, (uint32_t)need[npairs + it.pair]);
+            acq_lines = it.pair;
+        }
+        if (!ok) {
+            if (lane == 0) __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+        const BPair &P = pairs[it.pair];
+        if (it.kind == IT_BT) {
+            run_bt_item(P, it);
+        } else if (it.kind == IT_LINES) {
+            sgm_line_group_buf<M, true, PAD, PF, LPL>(P.a, it.a, it.b, it.a, prio);
+            __builtin_amdgcn_s_setprio(0);
+        } else {
+            run_sum_units<M>(P, it.a, it.b);
+        }
+        if (it.kind != IT_SUM) {
+            cur_pair = it.pair;
+            cur_kind = it.kind;
+            pending++;
+        }
     }
+    flush();
+}
+
+__global__ void batch_reset_kernel(uint32_t *ctl, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        ctl[i] = 0;
 }
 
 } // namespace tdvh
@@ -121,49 +222,36 @@ using namespace tdvh;
 struct tdvh_batch {
     tdvh_ctx *ctx = nullptr;
     int n = 0;
+    std::vector<tdvh_pair> host_pairs;
     tdvh_sgm_params params{};
-    void *mem = nullptr; // all device memory of the batch
+    void *mem = nullptr; // pair buffers, descriptors, control words
     size_t mem_bytes = 0;
     BPair *d_pairs = nullptr;
-    BItem *d_items = nullptr;
-    // item list ranges: BT rows, then per depth class m (1..8) lines and sums
-    int bt_begin = 0, bt_count = 0;
-    // segments: per depth class, consecutive groups of its pairs; a
-    // segment's plane sums overlap the next segment's line scans
-    struct Seg {
-        int mc;
-        int line_begin, line_count; // line-group items, longest first
-        int sum_begin, sum_count;   // sum items
-        int sum_waves;              // waves of the wide sum kernel
-    };
-    std::vector<Seg> segs;
-    bool class_pad[9] = {}; // some pair of the class has padding cells (D < 32m)
-    bool class_buf[9] = {}; // every pair of the class fits the buffer line kernel
+    uint32_t *d_ctl = nullptr;
+    int nctl = 0;
+    // item plan (rebuilt when lines per wave or the fusion depth change)
+    int *d_need = nullptr;    // per pair: BT items, then LINES items (own allocation)
+    BItem *d_items = nullptr; // the items, after the need counts
+    size_t items_cap = 0;
+    struct Class {
+        bool used = false, pad = false, buf = true;
+        int bt_begin = 0, bt_count = 0;     // depth 3: BT items of the class
+        int line_begin = 0, line_count = 0; // depth 1
+        int sum_begin = 0, sum_count = 0, sum_waves = 0;
+        int queue_begin = 0, queue_count = 0; // depth 2, 3
+    } cls[9];
+    int bt_begin = 0, bt_count = 0; // depth 1, 2: BT items of every pair
     int64_t cells = 0;
     // knobs
-    int waves_per_group = 1, prefetch = 8, prio = 1;
+    int lines_per_wave = 4, waves_per_group = 1, prefetch = 8, depth = 1, prio = 1;
+    int sum_lag = -1; // persistent queues: pairs between a pair's lines and its sums
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t graph_stream = nullptr;
     hipStream_t branch[9] = {}, sum_branch[9] = {};
-    hipEvent_t fork = nullptr, join[9] = {}, sum_join[9] = {};
-    std::vector<hipEvent_t> seg_done; // per segment: its line scans finished
+    hipEvent_t fork = nullptr, join[9] = {}, sum_join[9] = {}, lines_done[9] = {};
 };
-
-// Segments per depth class (TDVH_BATCH_SEGMENTS, default 1). Measured on the
-// 64-pair mix: overlapping a segment's plane sums with the next segment's
-// line scans (4 segments) ran ~10 % slower than one segment per class, as
-// both phases then compete for the same memory pipeline and HBM.
-static int batch_segments()
-{
-    static const int n = [] {
-        const char *e = getenv("TDVH_BATCH_SEGMENTS");
-        const int v = e ? atoi(e) : 1;
-        return v < 1 ? 1 : (v > 16 ? 16 : v);
-    }();
-    return n;
-}
 
 static void drop_graph(tdvh_batch *b)
 {
@@ -171,6 +259,137 @@ static void drop_graph(tdvh_batch *b)
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
+}
+
+// Line groups of one pair, longest first.
+static void pair_line_items(const tdvh_pair &p, int i, int lpw, std::vector<BItem> &out)
+{
+    std::vector<std::pair<int, BItem>> lines;
+    for (int dir = 0; dir < 8; dir++) {
+        const int nl = dir_lines(dir, p.width, p.height);
+        for (int l0 = 0; l0 < nl; l0 += lpw) {
+            int longest = 0;
+            for (int l = l0; l < std::min(l0 + lpw, nl); l++) {
+                int sx, sy, len;
+                line_geom(dir, l, p.width, p.height, sx, sy, len);
+                longest = std::max(longest, len);
+            }
+            lines.push_back({longest, BItem{i, IT_LINES, dir, l0}});
+        }
+    }
+    std::stable_sort(lines.begin(), lines.end(),
+                     [](const std::pair<int, BItem> &x, const std::pair<int, BItem> &y) {
+                         return x.first > y.first;
+                     });
+    for (auto &l : lines) out.push_back(l.second);
+}
+
+static int pair_sum_units(const tdvh_pair &p, int mc)
+{
+    const long npix = (long)p.width * p.height;
+    const int G = mc == 6 ? 16 : 2 * mc;
+    return (int)((npix * G + 63) / 64);
+}
+
+// Lines per wave usable for class mc: 8 needs 8-lane lines (2m words per
+// lane, at most 4, which keeps the prefetch ring within two waves per SIMD)
+// and the buffer kernel.
+static int class_lpw(const tdvh_batch *b, int mc)
+{
+    return (b->lines_per_wave == 8 && mc <= 2 && b->cls[mc].buf) ? 8 : 4;
+}
+
+// Builds the item lists for the current knobs and uploads them (the batch
+// is idle: callers synchronise first).
+static int plan_items(tdvh_batch *b)
+{
+    const int n = b->n;
+    std::vector<BItem> items;
+    std::vector<int> need(2 * n, 0);
+    auto bt_items = [&](int i) {
+        const int H = b->host_pairs[i].height;
+        for (int y = 0; y < H; y += BT_ROWS) {
+            items.push_back({i, IT_BT, y, std::min(BT_ROWS, H - y)});
+            need[i]++;
+        }
+    };
+    b->bt_begin = (int)items.size();
+    if (b->depth < 3)
+        for (int i = 0; i < n; i++) bt_items(i);
+    b->bt_count = (int)items.size() - b->bt_begin;
+    for (int mc = 1; mc <= 8; mc++) {
+        tdvh_batch::Class &c = b->cls[mc];
+        if (!c.used) continue;
+        std::vector<int> members;
+        for (int i = 0; i < n; i++)
+            if (scan_m(b->host_pairs[i].depth) == mc) members.push_back(i);
+        const int lpw = class_lpw(b, mc);
+        const bool persistent = b->depth >= 2 && c.buf;
+        if (!persistent) {
+            c.line_begin = (int)items.size();
+            for (int i : members) {
+                const size_t before = items.size();
+                pair_line_items(b->host_pairs[i], i, lpw, items);
+                need[n + i] = (int)(items.size() - before);
+            }
+            c.line_count = (int)items.size() - c.line_begin;
+            c.sum_begin = (int)items.size();
+            int wave = 0;
+            for (int i : members) {
+                const int nw = pair_sum_units(b->host_pairs[i], mc);
+                items.push_back({i, IT_SUM, wave, nw});
+                wave += nw;
+            }
+            c.sum_count = (int)items.size() - c.sum_begin;
+            c.sum_waves = wave;
+            c.queue_count = 0;
+            continue;
+        }
+        // one queue: [BT of the class's pairs (depth 3)], then pair by pair
+        // the line groups, each pair's sums after the next pair's lines
+        c.queue_begin = (int)items.size();
+        if (b->depth == 3)
+            for (int i : members) bt_items(i);
+        auto sums = [&](int i) {
+            const int units = pair_sum_units(b->host_pairs[i], mc);
+            for (int u = 0; u < units; u += SUM_UNITS_PERSIST)
+                items.push_back({i, IT_SUM, u, std::min(SUM_UNITS_PERSIST, units - u)});
+        };
+        // sum_lag: a pair's sums follow the line groups of that many later
+        // pairs (-1: all sums after all line groups of the class)
+        const int lag = b->sum_lag;
+        for (size_t k = 0; k < members.size(); k++) {
+            const int i = members[k];
+            const size_t before = items.size();
+            pair_line_items(b->host_pairs[i], i, lpw, items);
+            need[n + i] = (int)(items.size() - before);
+            if (lag > 0 && (int)k >= lag) sums(members[k - lag]);
+        }
+        for (size_t k = lag > 0 ? std::max<long>(0, (long)members.size() - lag) : 0;
+             k < members.size(); k++)
+            sums(members[k]);
+        c.queue_count = (int)items.size() - c.queue_begin;
+    }
+    // device copies: the need counts sit in front of the items (one
+    // allocation, grown when a plan needs more)
+    const size_t need_bytes = (need.size() * sizeof(int) + 255) & ~(size_t)255;
+    const size_t bytes = need_bytes + items.size() * sizeof(BItem);
+    if (bytes > b->items_cap) {
+        if (b->d_need) TDVH_CHECK_HIP(hipFree(b->d_need));
+        b->d_need = nullptr;
+        b->d_items = nullptr;
+        b->items_cap = 0;
+        void *p = nullptr;
+        TDVH_CHECK_HIP(hipMalloc(&p, bytes));
+        b->d_need = (int *)p;
+        b->items_cap = bytes;
+    }
+    b->d_items = (BItem *)((uint8_t *)b->d_need + need_bytes);
+    TDVH_CHECK_HIP(hipMemcpy(b->d_need, need.data(), need.size() * sizeof(int),
+                             hipMemcpyHostToDevice));
+    TDVH_CHECK_HIP(hipMemcpy(b->d_items, items.data(), items.size() * sizeof(BItem),
+                             hipMemcpyHostToDevice));
+    return TDVH_OK;
 }
 
 extern "C" int tdvh_batch_create(tdvh_ctx *ctx, const tdvh_pair *pairs, int n,
@@ -206,99 +425,11 @@ extern "C" int tdvh_batch_create(tdvh_ctx *ctx, const tdvh_pair *pairs, int n,
     }
     const size_t trash_off = total;
     total = align(total + 64 * 64);
-    // items: BT rows of all pairs, then per depth class the line groups
-    // (longest line first) and the pixel chunks
-    std::vector<BItem> items;
-    const int rows_per_item = 4;
-    for (int i = 0; i < n; i++)
-        for (int y = 0; y < pairs[i].height; y += rows_per_item)
-            items.push_back({i, 0, y, std::min(rows_per_item, pairs[i].height - y)});
-    tdvh_batch plan;
-    plan.bt_begin = 0;
-    plan.bt_count = (int)items.size();
-    for (int mc = 1; mc <= 8; mc++) {
-        std::vector<int> cls;
-        int64_t cls_cells = 0;
-        for (int i = 0; i < n; i++)
-            if (scan_m(pairs[i].depth) == mc) {
-                cls.push_back(i);
-                cls_cells += (int64_t)pairs[i].width * pairs[i].height * pairs[i].depth;
-            }
-        if (cls.empty()) continue;
-        // K consecutive groups of the class's pairs with about equal cells:
-        // group k ends where the running total reaches k+1 K-ths, leaving at
-        // least one pair to each later group
-        const int K = std::min((int)cls.size(), batch_segments());
-        size_t c0 = 0;
-        int64_t acc = 0;
-        for (int k = 0; k < K; k++) {
-            size_t c1 = c0;
-            const int64_t goal = cls_cells * (k + 1) / K;
-            const size_t max_end = cls.size() - (size_t)(K - k - 1);
-            do {
-                const tdvh_pair &q = pairs[cls[c1]];
-                acc += (int64_t)q.width * q.height * q.depth;
-                c1++;
-            } while (c1 < max_end && acc < goal);
-            tdvh_batch::Seg sg{mc, 0, 0, 0, 0, 0};
-            std::vector<std::pair<int, BItem>> lines;
-            for (size_t c = c0; c < c1; c++) {
-                const int i = cls[c];
-                const int W = pairs[i].width, H = pairs[i].height;
-                for (int dir = 0; dir < 8; dir++) {
-                    const int nl = dir_lines(dir, W, H);
-                    for (int l0 = 0; l0 < nl; l0 += 4) {
-                        int longest = 0;
-                        for (int l = l0; l < std::min(l0 + 4, nl); l++) {
-                            int sx, sy, len;
-                            line_geom(dir, l, W, H, sx, sy, len);
-                            longest = std::max(longest, len);
-                        }
-                        lines.push_back({longest, BItem{i, 1, dir, l0}});
-                    }
-                }
-            }
-            // pair-major, longest line first within a pair: the next pair's
-            // long lines start while this pair's short ones finish, and only
-            // about one pair's term planes are live in the caches at a time
-            std::stable_sort(lines.begin(), lines.end(),
-                             [](const std::pair<int, BItem> &x, const std::pair<int, BItem> &y) {
-                                 if (x.second.pair != y.second.pair)
-                                     return x.second.pair < y.second.pair;
-                                 return x.first > y.first;
-                             });
-            sg.line_begin = (int)items.size();
-            for (auto &l : lines) items.push_back(l.second);
-            sg.line_count = (int)items.size() - sg.line_begin;
-            sg.sum_begin = (int)items.size();
-            if (mc == 1 || mc == 2 || mc == 4 || mc == 6 || mc == 8) {
-                // one item per pair: {pair, 3, first wave, waves} (wide sum)
-                int wave = 0;
-                for (size_t c = c0; c < c1; c++) {
-                    const int i = cls[c];
-                    const long npix = (long)pairs[i].width * pairs[i].height;
-                    const int nw = (int)((npix * (mc == 6 ? 16 : 2 * mc) + 63) / 64);
-                    items.push_back({i, 3, wave, nw});
-                    wave += nw;
-                }
-                sg.sum_waves = wave;
-            } else {
-                for (size_t c = c0; c < c1; c++) {
-                    const int i = cls[c];
-                    const long npix = (long)pairs[i].width * pairs[i].height;
-                    for (long p = 0; p < npix; p += SUM_CHUNK)
-                        items.push_back({i, 2, (int)p, (int)std::min<long>(SUM_CHUNK, npix - p)});
-                }
-            }
-            sg.sum_count = (int)items.size() - sg.sum_begin;
-            plan.segs.push_back(sg);
-            c0 = c1;
-        }
-    }
     const size_t pairs_off = total;
     total = align(total + sizeof(BPair) * n);
-    const size_t items_off = total;
-    total = align(total + sizeof(BItem) * items.size());
+    const size_t ctl_off = total;
+    const int nctl = CTL_PAIRS + 2 * n;
+    total = align(total + 4 * (size_t)nctl);
 
     tdvh_batch *b = new tdvh_batch();
     hipError_t e = hipMalloc(&b->mem, total);
@@ -335,27 +466,30 @@ extern "C" int tdvh_batch_create(tdvh_ctx *ctx, const tdvh_pair *pairs, int n,
     }
     b->ctx = ctx;
     b->n = n;
+    b->host_pairs.assign(pairs, pairs + n);
     b->params = *params;
     b->mem_bytes = total;
     b->d_pairs = (BPair *)(base + pairs_off);
-    b->d_items = (BItem *)(base + items_off);
-    b->bt_begin = plan.bt_begin;
-    b->bt_count = plan.bt_count;
-    b->segs = plan.segs;
-    for (int mc = 1; mc <= 8; mc++) b->class_buf[mc] = true;
+    b->d_ctl = (uint32_t *)(base + ctl_off);
+    b->nctl = nctl;
     for (int i = 0; i < n; i++) {
-        const int mc = bp[i].m;
-        if (bp[i].a.D < 32 * mc) b->class_pad[mc] = true;
-        if (!buf_lines_fit(bp[i].a)) b->class_buf[mc] = false;
+        tdvh_batch::Class &c = b->cls[bp[i].m];
+        c.used = true;
+        if (bp[i].a.D < 32 * bp[i].m) c.pad = true;
+        if (!buf_lines_fit(bp[i].a)) c.buf = false;
     }
     b->cells = cells;
-    if (hipMemcpy(b->d_pairs, bp.data(), sizeof(BPair) * n, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(b->d_items, items.data(), sizeof(BItem) * items.size(),
-                  hipMemcpyHostToDevice) != hipSuccess) {
+    int rc = TDVH_OK;
+    if (hipMemcpy(b->d_pairs, bp.data(), sizeof(BPair) * n, hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("uploading the batch descriptors failed");
+        rc = TDVH_ERR_DEVICE;
+    }
+    if (rc == TDVH_OK) rc = plan_items(b);
+    if (rc != TDVH_OK) {
+        if (b->d_need) (void)hipFree(b->d_need);
         (void)hipFree(b->mem);
         delete b;
-        set_error("uploading the batch plan failed");
-        return TDVH_ERR_DEVICE;
+        return rc;
     }
     *out = b;
     return TDVH_OK;
@@ -374,59 +508,82 @@ extern "C" int tdvh_batch_destroy(tdvh_batch *b)
         if (b->sum_branch[mc]) (void)hipStreamDestroy(b->sum_branch[mc]);
         if (b->join[mc]) (void)hipEventDestroy(b->join[mc]);
         if (b->sum_join[mc]) (void)hipEventDestroy(b->sum_join[mc]);
+        if (b->lines_done[mc]) (void)hipEventDestroy(b->lines_done[mc]);
     }
-    for (hipEvent_t e : b->seg_done)
-        if (e) (void)hipEventDestroy(e);
     if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->d_need) (void)hipFree(b->d_need);
     (void)hipFree(b->mem);
     delete b;
     return TDVH_OK;
 }
 
 extern "C" int tdvh_batch_config(tdvh_batch *b, int lines_per_wave, int waves_per_group,
-                                 int prefetch)
+                                 int prefetch, int fusion_depth, int sum_lag)
 {
     clear_error();
     TDVH_REQUIRE(b, "null batch");
-    TDVH_REQUIRE(lines_per_wave == 0 || lines_per_wave == 4, "lines per wave is 4");
+    TDVH_TRY(use_device(b->ctx));
+    TDVH_REQUIRE(lines_per_wave == 0 || lines_per_wave == 4 || lines_per_wave == 8,
+                 "lines per wave must be 4 or 8");
     TDVH_REQUIRE(waves_per_group == 0 || waves_per_group == 1 || waves_per_group == 2 ||
                      waves_per_group == 4,
                  "waves per group must be 1, 2 or 4");
     TDVH_REQUIRE(prefetch == 0 || prefetch == 4 || prefetch == 8, "prefetch must be 4 or 8");
+    TDVH_REQUIRE(fusion_depth >= 0 && fusion_depth <= 3, "fusion depth must be 1, 2 or 3");
+    TDVH_REQUIRE(sum_lag >= -1, "sum lag must be -1 or at least 0");
+    const int lpw = lines_per_wave ? lines_per_wave : b->lines_per_wave;
     const int wpg = waves_per_group ? waves_per_group : b->waves_per_group;
     const int pf = prefetch ? prefetch : b->prefetch;
-    if (wpg != b->waves_per_group || pf != b->prefetch) drop_graph(b);
+    const int dep = fusion_depth ? fusion_depth : b->depth;
+    const int lag = sum_lag ? sum_lag : b->sum_lag;
+    const bool replan = lpw != b->lines_per_wave || dep != b->depth || lag != b->sum_lag;
+    if (replan || wpg != b->waves_per_group || pf != b->prefetch) drop_graph(b);
+    b->lines_per_wave = lpw;
     b->waves_per_group = wpg;
     b->prefetch = pf;
+    b->depth = dep;
+    b->sum_lag = lag;
+    if (replan) {
+        // queued runs may still read the previous item lists
+        TDVH_CHECK_HIP(hipDeviceSynchronize());
+        TDVH_TRY(plan_items(b));
+    }
     return TDVH_OK;
 }
 
+template <int M, bool PAD, int PF>
+static void launch_lines_buf(tdvh_batch *b, int lpl, const BItem *li, int nl, int blocks,
+                             int threads, hipStream_t s)
+{
+    if constexpr (M <= 2) {
+        if (lpl == 8) {
+            batch_lines_buf_kernel<M, PAD, PF, 8><<<blocks, threads, 0, s>>>(b->d_pairs, li, nl,
+                                                                             b->prio);
+            return;
+        }
+    }
+    batch_lines_buf_kernel<M, PAD, PF, 16><<<blocks, threads, 0, s>>>(b->d_pairs, li, nl, b->prio);
+}
+
 template <int M>
-static void launch_lines_seg(tdvh_batch *b, const tdvh_batch::Seg &sg, hipStream_t s)
+static void launch_lines_class(tdvh_batch *b, const tdvh_batch::Class &c, int mc, hipStream_t s)
 {
     const int threads = 64 * b->waves_per_group, wpb = b->waves_per_group;
-    const int nl = sg.line_count;
+    const int nl = c.line_count;
     if (!nl) return;
-    const BItem *li = b->d_items + sg.line_begin;
+    const BItem *li = b->d_items + c.line_begin;
     const int blocks = (nl + wpb - 1) / wpb;
+    const int lpl = 64 / class_lpw(b, mc);
     prof_begin(s);
-    if (b->class_buf[M]) {
+    if (c.buf) {
         // "prefetch 8": the deepest ring that keeps two waves per SIMD
         constexpr int PF8 = buf_lines_pf<M, true>();
-        if (b->class_pad[M]) {
-            if (b->prefetch == 8)
-                batch_lines_buf_kernel<M, true, PF8><<<blocks, threads, 0, s>>>(b->d_pairs, li, nl,
-                                                                                b->prio);
-            else
-                batch_lines_buf_kernel<M, true, 4><<<blocks, threads, 0, s>>>(b->d_pairs, li, nl,
-                                                                              b->prio);
+        if (c.pad) {
+            if (b->prefetch == 8) launch_lines_buf<M, true, PF8>(b, lpl, li, nl, blocks, threads, s);
+            else launch_lines_buf<M, true, 4>(b, lpl, li, nl, blocks, threads, s);
         } else {
-            if (b->prefetch == 8)
-                batch_lines_buf_kernel<M, false, PF8><<<blocks, threads, 0, s>>>(b->d_pairs, li,
-                                                                                 nl, b->prio);
-            else
-                batch_lines_buf_kernel<M, false, 4><<<blocks, threads, 0, s>>>(b->d_pairs, li, nl,
-                                                                               b->prio);
+            if (b->prefetch == 8) launch_lines_buf<M, false, PF8>(b, lpl, li, nl, blocks, threads, s);
+            else launch_lines_buf<M, false, 4>(b, lpl, li, nl, blocks, threads, s);
         }
     } else {
         if (b->prefetch == 8)
@@ -438,85 +595,124 @@ static void launch_lines_seg(tdvh_batch *b, const tdvh_batch::Seg &sg, hipStream
 }
 
 template <int M>
-static void launch_sum_seg(tdvh_batch *b, const tdvh_batch::Seg &sg, hipStream_t s)
+static void launch_sum_class(tdvh_batch *b, const tdvh_batch::Class &c, hipStream_t s)
 {
-    const int ns = sg.sum_count;
-    if (!ns) return;
-    const BItem *si = b->d_items + sg.sum_begin;
+    if (!c.sum_count) return;
+    const BItem *si = b->d_items + c.sum_begin;
     prof_begin(s);
-    if constexpr (sum_wide_ok<M>())
-        batch_sum_wide_kernel<M><<<(sg.sum_waves + 3) / 4, 256, 0, s>>>(b->d_pairs, si, ns,
-                                                                        sg.sum_waves);
-    else
-        batch_sum_kernel<M><<<(ns + 3) / 4, 256, 0, s>>>(b->d_pairs, si, ns);
+    batch_sum_wide_kernel<M><<<(c.sum_waves + 3) / 4, 256, 0, s>>>(b->d_pairs, si, c.sum_count,
+                                                                    c.sum_waves);
     (void)check_launch("batch_sum_kernel", s);
 }
 
-static int launch_seg(tdvh_batch *b, const tdvh_batch::Seg &sg, bool sum, hipStream_t s)
+template <int M, bool PAD, int PF>
+static void launch_queue_pf(tdvh_batch *b, int lpl, const tdvh_batch::Class &c, int mc, int grid,
+                            int threads, hipStream_t s)
 {
-    switch (sg.mc) {
-#define CASE(m)                                                                                    \
-    case m:                                                                                        \
-        if (sum) launch_sum_seg<m>(b, sg, s);                                                      \
-        else launch_lines_seg<m>(b, sg, s);                                                        \
-        break;
-        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
-#undef CASE
-    default: break;
+    const BItem *qi = b->d_items + c.queue_begin;
+    const int wait_bt = b->depth == 3;
+    if constexpr (M <= 2) {
+        if (lpl == 8) {
+            batch_queue_kernel<M, PAD, PF, 8><<<grid, threads, 0, s>>>(
+                b->d_pairs, qi, c.queue_count, b->d_need, b->d_ctl, b->n, mc, wait_bt, b->prio);
+            return;
+        }
     }
-    return check_launch(sum ? "batch_sum" : "batch_lines", s);
+    batch_queue_kernel<M, PAD, PF, 16><<<grid, threads, 0, s>>>(
+        b->d_pairs, qi, c.queue_count, b->d_need, b->d_ctl, b->n, mc, wait_bt, b->prio);
 }
 
-// Enqueues the batch on `s`. With fork (graph capture) every depth class
-// runs on two branch streams forked from and joined back into `s`: one scans
-// the lines of its segments in order, the other sums each segment as soon as
-// its lines are done, overlapping the next segment's scans (VALU-bound) with
-// this segment's plane reads (HBM-bound).
+// One persistent launch for the class: as many waves as stay resident at two
+// per SIMD (fewer for short queues).
+template <int M>
+static void launch_queue_class(tdvh_batch *b, const tdvh_batch::Class &c, int mc, hipStream_t s)
+{
+    if (!c.queue_count) return;
+    const int wpb = b->waves_per_group, threads = 64 * wpb;
+    const int resident = 2 * 4 * std::max(b->ctx->num_cus, 1);
+    const int waves = std::min(c.queue_count, resident);
+    const int grid = (waves + wpb - 1) / wpb;
+    const int lpl = 64 / class_lpw(b, mc);
+    constexpr int PF8 = buf_lines_pf<M, true>();
+    prof_begin(s);
+    if (c.pad) {
+        if (b->prefetch == 8) launch_queue_pf<M, true, PF8>(b, lpl, c, mc, grid, threads, s);
+        else launch_queue_pf<M, true, 4>(b, lpl, c, mc, grid, threads, s);
+    } else {
+        if (b->prefetch == 8) launch_queue_pf<M, false, PF8>(b, lpl, c, mc, grid, threads, s);
+        else launch_queue_pf<M, false, 4>(b, lpl, c, mc, grid, threads, s);
+    }
+    (void)check_launch("batch_queue_kernel", s);
+}
+
+// Phase `what` (0 lines, 1 sums, 2 persistent queue) of class mc on `s`.
+static int launch_class(tdvh_batch *b, int mc, int what, hipStream_t s)
+{
+    const tdvh_batch::Class &c = b->cls[mc];
+    switch (mc) {
+#define CASE(m)                                                                                    \
+    case m:                                                                                        \
+        if (what == 0) launch_lines_class<m>(b, c, mc, s);                                         \
+        else if (what == 1) launch_sum_class<m>(b, c, s);                                          \
+        else launch_queue_class<m>(b, c, mc, s);                                                   \
+        break;
+        CASE(1) CASE(2) CASE(4) CASE(6) CASE(8)
+#undef CASE
+    default: set_error("no batch kernels for %d words per lane", mc); return TDVH_ERR_UNSUPPORTED;
+    }
+    return check_launch("batch_class", s);
+}
+
+static bool class_persistent(const tdvh_batch *b, int mc)
+{
+    return b->depth >= 2 && b->cls[mc].buf;
+}
+
+// Enqueues the batch on `s`. With fork (graph capture) each depth class runs
+// on branch streams forked from and joined back into `s`.
 static int enqueue_batch(tdvh_batch *b, hipStream_t s, bool fork)
 {
-    prof_begin(s);
-    batch_bt_kernel<<<(b->bt_count + 3) / 4, 256, 0, s>>>(b->d_pairs, b->d_items + b->bt_begin,
-                                                          b->bt_count);
-    TDVH_TRY(check_launch("batch_bt_kernel", s));
+    if (b->depth >= 2) {
+        prof_begin(s);
+        batch_reset_kernel<<<1, 256, 0, s>>>(b->d_ctl, b->nctl);
+        TDVH_TRY(check_launch("batch_reset_kernel", s));
+    }
+    if (b->bt_count) {
+        prof_begin(s);
+        batch_bt_kernel<<<(b->bt_count + 3) / 4, 256, 0, s>>>(b->d_pairs, b->d_items + b->bt_begin,
+                                                              b->bt_count);
+        TDVH_TRY(check_launch("batch_bt_kernel", s));
+    }
     if (!fork) {
-        for (const auto &sg : b->segs) {
-            TDVH_TRY(launch_seg(b, sg, false, s));
-            TDVH_TRY(launch_seg(b, sg, true, s));
+        for (int mc = 1; mc <= 8; mc++) {
+            if (!b->cls[mc].used) continue;
+            if (class_persistent(b, mc)) {
+                TDVH_TRY(launch_class(b, mc, 2, s));
+            } else {
+                TDVH_TRY(launch_class(b, mc, 0, s));
+                TDVH_TRY(launch_class(b, mc, 1, s));
+            }
         }
         return TDVH_OK;
     }
     if (!b->fork) TDVH_CHECK_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
-    if (b->seg_done.size() < b->segs.size()) b->seg_done.resize(b->segs.size(), nullptr);
     TDVH_CHECK_HIP(hipEventRecord(b->fork, s));
     for (int mc = 1; mc <= 8; mc++) {
-        bool any = false;
-        for (const auto &sg : b->segs) any |= sg.mc == mc;
-        if (!any) continue;
+        if (!b->cls[mc].used) continue;
         if (!b->branch[mc])
             TDVH_CHECK_HIP(hipStreamCreateWithFlags(&b->branch[mc], hipStreamNonBlocking));
-        if (!b->sum_branch[mc])
-            TDVH_CHECK_HIP(hipStreamCreateWithFlags(&b->sum_branch[mc], hipStreamNonBlocking));
         if (!b->join[mc])
             TDVH_CHECK_HIP(hipEventCreateWithFlags(&b->join[mc], hipEventDisableTiming));
-        if (!b->sum_join[mc])
-            TDVH_CHECK_HIP(hipEventCreateWithFlags(&b->sum_join[mc], hipEventDisableTiming));
-        hipStream_t ls = b->branch[mc], ss = b->sum_branch[mc];
+        hipStream_t ls = b->branch[mc];
         TDVH_CHECK_HIP(hipStreamWaitEvent(ls, b->fork, 0));
-        TDVH_CHECK_HIP(hipStreamWaitEvent(ss, b->fork, 0));
-        for (size_t k = 0; k < b->segs.size(); k++) {
-            const auto &sg = b->segs[k];
-            if (sg.mc != mc) continue;
-            if (!b->seg_done[k])
-                TDVH_CHECK_HIP(hipEventCreateWithFlags(&b->seg_done[k], hipEventDisableTiming));
-            TDVH_TRY(launch_seg(b, sg, false, ls));
-            TDVH_CHECK_HIP(hipEventRecord(b->seg_done[k], ls));
-            TDVH_CHECK_HIP(hipStreamWaitEvent(ss, b->seg_done[k], 0));
-            TDVH_TRY(launch_seg(b, sg, true, ss));
+        if (class_persistent(b, mc)) {
+            TDVH_TRY(launch_class(b, mc, 2, ls));
+        } else {
+            TDVH_TRY(launch_class(b, mc, 0, ls));
+            TDVH_TRY(launch_class(b, mc, 1, ls));
         }
         TDVH_CHECK_HIP(hipEventRecord(b->join[mc], ls));
-        TDVH_CHECK_HIP(hipEventRecord(b->sum_join[mc], ss));
         TDVH_CHECK_HIP(hipStreamWaitEvent(s, b->join[mc], 0));
-        TDVH_CHECK_HIP(hipStreamWaitEvent(s, b->sum_join[mc], 0));
     }
     return TDVH_OK;
 }
@@ -553,6 +749,11 @@ extern "C" int tdvh_batch_wait(tdvh_batch *b, void *stream)
     TDVH_REQUIRE(b, "null batch");
     TDVH_TRY(use_device(b->ctx));
     TDVH_CHECK_HIP(hipStreamSynchronize((hipStream_t)stream));
+    if (b->depth >= 2) {
+        uint32_t err = 0;
+        TDVH_CHECK_HIP(hipMemcpy(&err, b->d_ctl + CTL_ERR, 4, hipMemcpyDeviceToHost));
+        TDVH_REQUIRE(err == 0, "a batch work queue stalled (a wait hit its spin limit)");
+    }
     return TDVH_OK;
 }
 

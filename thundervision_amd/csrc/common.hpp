@@ -94,6 +94,9 @@ struct tdvh_ctx {
         uint64_t tag = 0; // what the contents were built for (0: nothing)
     };
     std::unordered_map<std::string, Slot> slots;
+    // host copies of tables uploaded asynchronously into slots (kept as
+    // long as the context, so an in-flight copy never loses its source)
+    std::unordered_map<std::string, std::vector<uint32_t>> host_tables;
 
     // stage timing
     struct ProfMark {

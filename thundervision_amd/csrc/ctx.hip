@@ -117,6 +117,7 @@ int tdvh_ctx::release_all()
     for (auto &kv : slots)
         if (kv.second.ptr) TDVH_CHECK_HIP(hipFree(kv.second.ptr));
     slots.clear();
+    host_tables.clear(); // their copies completed at the synchronisation above
     return TDVH_OK;
 }
 

@@ -153,7 +153,7 @@ constexpr int NXCD = 8;
 // Entry b is the next entry of chiplet b % NXCD's longest-first queue.
 // Built and uploaded once per (W, H, dir_mask) and context.
 static int line_order(tdvh_ctx *ctx, int W, int H, int dir_mask, int lpw, const uint32_t **out,
-                      int *groups)
+                      int *groups, hipStream_t s)
 {
     TDVH_REQUIRE(W + H < (1 << 24), "image %dx%d too large for the line table", W, H);
     struct Group {
@@ -212,8 +212,10 @@ static int line_order(tdvh_ctx *ctx, int W, int H, int dir_mask, int lpw, const 
         }
         h.push_back(q[c][head[c]++].second);
     }
-    // one table per shape and mask (a context serving several shapes keeps
-    // them all, so steady-state calls never upload or synchronise)
+    // one table per shape, mask and lines per wave: a context serving several
+    // shapes keeps them all, each in its own buffer that no other shape
+    // reuses, so the upload is ordered on the launch stream (no device-wide
+    // synchronisation) and steady-state calls never upload
     char name[64];
     snprintf(name, sizeof name, "sgm_line_order:%dx%d:%02x:%d", W, H, dir_mask, lpw);
     void *buf = nullptr;
@@ -222,13 +224,17 @@ static int line_order(tdvh_ctx *ctx, int W, int H, int dir_mask, int lpw, const 
     const uint64_t tag = ((uint64_t)W << 40) | ((uint64_t)H << 16) | ((uint64_t)lpw << 8) |
                          (uint64_t)dir_mask;
     if (sl.tag != tag) {
-        // queued launches may still read the previous table
-        TDVH_CHECK_HIP(hipDeviceSynchronize());
-        TDVH_CHECK_HIP(hipMemcpy(buf, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+        // the host copy lives in the context for as long as the buffer, so
+        // the asynchronous copy's source stays valid
+        std::vector<uint32_t> &host = ctx->host_tables[name];
+        host = std::move(h);
+        TDVH_CHECK_HIP(hipMemcpyAsync(buf, host.data(), host.size() * 4, hipMemcpyHostToDevice, s));
         sl.tag = tag;
+        *groups = (int)host.size();
+    } else {
+        *groups = (int)h.size();
     }
     *out = (const uint32_t *)buf;
-    *groups = (int)h.size();
     return TDVH_OK;
 }
 
@@ -242,7 +248,7 @@ int launch_sgm_lines(tdvh_ctx *ctx, const SgmArgs &a, int dir_mask, bool eout, h
         const bool fused = a.C == nullptr;
         const bool lanes8 = lines_8lane(a);
         const int lpw = lanes8 ? 8 : 4; // lines per wave
-        TDVH_TRY(line_order(ctx, a.W, a.H, dir_mask, lpw, &order, &groups));
+        TDVH_TRY(line_order(ctx, a.W, a.H, dir_mask, lpw, &order, &groups, s));
         prof_begin(s);
         switch (a.m) {
 #define CASE(m)                                                                                    \

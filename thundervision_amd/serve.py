@@ -56,7 +56,8 @@ class Scheduler:
     the completion time (seconds on the schedule clock) or None;
     executor.now() is the schedule clock.
 
-    Admission is in arrival order, needs a free slot and memory within
+    Admission is in arrival order (largest_first: the largest queued frame
+    first, then arrival order), needs a free slot and memory within
     budget_bytes, and, with cell_budget, keeps the cells (W*H*D) of the
     frames in flight within that budget, so that large frames do not share
     the device with each other (a frame always starts on an idle device).
@@ -64,12 +65,14 @@ class Scheduler:
     start first while the head has waited less than backfill_s seconds.
     """
 
-    def __init__(self, executor, slots, budget_bytes, cell_budget=None, backfill_s=None):
+    def __init__(self, executor, slots, budget_bytes, cell_budget=None, backfill_s=None,
+                 largest_first=False):
         self.ex = executor
         self.nslots = slots
         self.budget = budget_bytes
         self.cell_budget = cell_budget
         self.backfill_s = backfill_s
+        self.largest_first = largest_first
         self.in_use = 0
         self.in_flight_cells = 0
         self.peak_in_use = 0
@@ -110,8 +113,11 @@ class Scheduler:
                                      % (f.width, f.height, f.depth))
                 queue.append(f)
                 i += 1
-            # admit in arrival order while a slot is free and the budgets
-            # allow (optionally backfilling past a blocked head frame)
+            # admit in arrival order (or largest frame first) while a slot is
+            # free and the budgets allow (optionally backfilling past a
+            # blocked head frame)
+            if self.largest_first and len(queue) > 1:
+                queue = collections.deque(sorted(queue, key=lambda f: (-f.cells, f.t_arrival)))
             while queue and free:
                 pick = None
                 for j, f in enumerate(queue):
@@ -147,6 +153,16 @@ class Scheduler:
             if not progressed and spin_sleep:
                 time.sleep(spin_sleep)
         return done
+
+
+def latency_by_shape(frames):
+    """latency_stats per (width, height, depth)."""
+    shapes = sorted({(f.width, f.height, f.depth) for f in frames})
+    out = {}
+    for sh in shapes:
+        st = latency_stats([f for f in frames if (f.width, f.height, f.depth) == sh])
+        out["%dx%d/D%d" % sh] = {k: st[k] for k in ("frames", "p50_ms", "p99_ms", "max_ms")}
+    return out
 
 
 def latency_stats(frames):

@@ -100,7 +100,7 @@ SIGNATURES = {
     "tdvh_batch_create": (_I, [_VP, ct.POINTER(Pair), _I, ct.POINTER(SgmParams),
                                ct.POINTER(_VP)]),
     "tdvh_batch_destroy": (_I, [_VP]),
-    "tdvh_batch_config": (_I, [_VP, _I, _I, _I]),
+    "tdvh_batch_config": (_I, [_VP, _I, _I, _I, _I, _I]),
     "tdvh_batch_run": (_I, [_VP, _I, _VP]),
     "tdvh_batch_wait": (_I, [_VP, _VP]),
     "tdvh_batch_cells": (_I64, [_VP]),
@@ -499,7 +499,8 @@ def sgm_paths(width, height):
 
 
 class Batch:
-    """Many pairs in one persistent launch (tdvh_batch_*)."""
+    """Many pairs as one graph of persistent (or per-phase) launches
+    (tdvh_batch_*)."""
 
     def __init__(self, pairs, p1=8, p2=32, ctx=None):
         self._keep = pairs  # (left, right, depth, disp) tuples keep tensors alive
@@ -514,8 +515,11 @@ class Batch:
                                        ct.byref(h)))
         self.handle = h
 
-    def config(self, lines_per_wave=0, waves_per_group=0, prefetch=0):
-        _check(lib().tdvh_batch_config(self.handle, lines_per_wave, waves_per_group, prefetch))
+    def config(self, lines_per_wave=0, waves_per_group=0, prefetch=0, fusion_depth=0,
+               sum_lag=0):
+        """Knobs of tdvh_batch_config (0 keeps the current value)."""
+        _check(lib().tdvh_batch_config(self.handle, lines_per_wave, waves_per_group, prefetch,
+                                       fusion_depth, sum_lag))
 
     def run(self, use_graph=True, stream=None):
         _check(lib().tdvh_batch_run(self.handle, int(use_graph), _stream(stream)))
