@@ -164,6 +164,23 @@ def test_dp_bounded_extreme_costs(tv, dev, seed):
     assert (tv.dp(Cx, p_occ, tv.DP_COST_BOUNDED).cpu().numpy() == want).all()
 
 
+@pytest.mark.parametrize("D", [64, 256, 512])
+def test_dp_bounded_renormalisation(tv, dev, D):
+    """Every valid cost just below 2^22: path sums pass 2^32 after about
+    1000 columns, so the 32-bit form is exact only if its periodic wave-
+    minimum subtraction works (a wrong or stale minimum saturates or wraps
+    the sums and changes the decisions)."""
+    rng = np.random.default_rng(D)
+    H, W = 4, 1700
+    C = ((1 << 22) - 1 - rng.integers(0, 3000, (H, W, D))).astype(np.uint32)
+    x = np.arange(W)[None, :, None]
+    d = np.arange(D)[None, None, :]
+    C[np.broadcast_to(d > x, C.shape)] = 0xFFFFFFFF
+    want = oracle.dp(C, 700)
+    got = tv.dp(to_dev(C.view(np.int32), dev), 700, tv.DP_COST_BOUNDED).cpu().numpy()
+    assert (got == want).all()
+
+
 def test_dp_bounded_flag_with_large_penalty_runs_64_bit(tv, dev):
     # the 32-bit form needs p_occ < 2^20; above it the flag is ignored
     rng = np.random.default_rng(12)

@@ -164,6 +164,40 @@ __device__ __forceinline__ int32_t row_min_i32(int32_t v)
     return (int32_t)(row_min_u32((uint32_t)v ^ 0x80000000u) ^ 0x80000000u);
 }
 
+// Raw buffer resources: 32-bit byte offsets from a base, range-checked
+// against a byte count (loads at or past it return 0, stores there are
+// dropped).
+typedef __amdgpu_buffer_rsrc_t buf_t;
+typedef unsigned int tdvh_u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int tdvh_u32x3 __attribute__((ext_vector_type(3)));
+typedef unsigned int tdvh_u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ buf_t buf_rsrc(const void *base, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes,
+                                             0x00020000);
+}
+
+// N consecutive dwords at byte offset off (0 outside the buffer); POL is
+// the cache-policy operand (2: non-temporal, for data read once).
+template <int N, int I = 0, int POL = 0>
+__device__ __forceinline__ void buf_load(buf_t r, uint32_t off, uint32_t *dst)
+{
+    if constexpr (N - I >= 4) {
+        const tdvh_u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off + 4 * I, 0, POL);
+        dst[I] = v.x, dst[I + 1] = v.y, dst[I + 2] = v.z, dst[I + 3] = v.w;
+        buf_load<N, I + 4, POL>(r, off, dst);
+    } else if constexpr (N - I == 3) {
+        const tdvh_u32x3 v = __builtin_amdgcn_raw_buffer_load_b96(r, off + 4 * I, 0, POL);
+        dst[I] = v.x, dst[I + 1] = v.y, dst[I + 2] = v.z;
+    } else if constexpr (N - I == 2) {
+        const tdvh_u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, off + 4 * I, 0, POL);
+        dst[I] = v.x, dst[I + 1] = v.y;
+    } else if constexpr (N - I == 1) {
+        dst[I] = __builtin_amdgcn_raw_buffer_load_b32(r, off + 4 * I, 0, POL);
+    }
+}
+
 // Sum over the 16 lanes of a DPP row; every lane receives it.
 __device__ __forceinline__ uint32_t row_sum_u32(uint32_t v)
 {

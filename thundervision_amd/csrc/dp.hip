@@ -37,13 +37,16 @@
 // input.
 //
 // Mapping: one wave per row, lane l holds d = l*K + k (K <= 8) in registers;
-// the d-1 / d+1 neighbours across lanes move with whole-wave DPP shifts (the
-// vacated end lane reads the invalid value, which is exactly the missing
-// neighbour of d = 0 and of d = 64K-1; cells d >= D carry invalid costs and
-// stay invalid, so no disparity bound needs a test). Costs stream through a
-// PF-step prefetch ring (one wave per SIMD leaves nothing else to hide the
-// load latency). Each step yields the lane's K two-bit decisions as 8 bits
-// (K <= 4) or 16 bits (K <= 8); CPW columns' bits are stored together as one
+// the penalised neighbours A(d-1) + P and A(d+1) + P across lanes move with
+// whole-wave DPP shifts (the vacated end lane keeps the invalid value, which
+// is exactly the missing neighbour of d = 0 and of d = 64K-1; cells d >= D
+// carry invalid costs and stay invalid, so no disparity bound needs a test).
+// Costs stream through a PF-step prefetch ring (one wave per SIMD leaves
+// nothing else to hide the load latency) addressed by 32-bit range-checked
+// buffer offsets. A single wave issues at most one instruction per cycle of
+// its SIMD's issue turn, so the step is written for instruction count: the
+// two decision bits of a cell (the comparison results) are shifted into the
+// decision word with one add each (2K bits per column); CPW columns fill one
 // word per lane (path word (x/CPW)*64 + lane), so that stores, which share
 // the memory counter with the loads, take few of its slots. Backtrack: per
 // block of 64 columns each lane loads its block's decision words together,
@@ -80,39 +83,34 @@ __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src)
 }
 
 // Whole-wave DPP shifts: wave_shr:1 (lane i reads lane i-1) and wave_shl:1
-// (lane i reads lane i+1); the lane without a source reads the invalid value.
-__device__ __forceinline__ uint64_t wave_from_prev(uint64_t v)
+// (lane i reads lane i+1); the lane without a source keeps `old`.
+template <int CTRL> __device__ __forceinline__ uint32_t wave_dpp(uint32_t old, uint32_t v)
 {
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)(uint32_t)v, 0x138, 0xF, 0xF, false);
-    const uint32_t hi =
-        (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)(uint32_t)(v >> 32), 0x138, 0xF, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL> __device__ __forceinline__ uint64_t wave_dpp(uint64_t old, uint64_t v)
+{
+    const uint32_t lo = wave_dpp<CTRL>((uint32_t)old, (uint32_t)v);
+    const uint32_t hi = wave_dpp<CTRL>((uint32_t)(old >> 32), (uint32_t)(v >> 32));
     return ((uint64_t)hi << 32) | lo;
 }
-__device__ __forceinline__ uint64_t wave_from_next(uint64_t v)
+template <int CTRL> __device__ __forceinline__ float wave_dpp(float old, float v)
 {
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)(uint32_t)v, 0x130, 0xF, 0xF, false);
-    const uint32_t hi =
-        (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)(uint32_t)(v >> 32), 0x130, 0xF, 0xF, false);
-    return ((uint64_t)hi << 32) | lo;
+    return __builtin_bit_cast(float, wave_dpp<CTRL>(__builtin_bit_cast(uint32_t, old),
+                                                    __builtin_bit_cast(uint32_t, v)));
 }
-__device__ __forceinline__ uint32_t wave_from_prev(uint32_t v)
+template <typename T> __device__ __forceinline__ T wave_from_prev(T old, T v) { return wave_dpp<0x138>(old, v); }
+template <typename T> __device__ __forceinline__ T wave_from_next(T old, T v) { return wave_dpp<0x130>(old, v); }
+
+// 2*cw + b in one add with carry-in: the lane mask of b is the comparison
+// result itself (no 0/1 materialisation, no shift).
+__device__ __forceinline__ uint32_t push_bit(uint32_t cw, bool b)
 {
-    return (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x138, 0xF, 0xF, false);
-}
-__device__ __forceinline__ uint32_t wave_from_next(uint32_t v)
-{
-    return (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x130, 0xF, 0xF, false);
-}
-constexpr int F32_INF_BITS = 0x7F800000;
-__device__ __forceinline__ float wave_from_prev(float v)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
-                                         F32_INF_BITS, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, false));
-}
-__device__ __forceinline__ float wave_from_next(float v)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
-                                         F32_INF_BITS, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, false));
+    const uint64_t m = __builtin_amdgcn_ballot_w64(b);
+    uint32_t r;
+    uint64_t carry_out;
+    asm("v_addc_co_u32 %0, %1, %2, %2, %3" : "=v"(r), "=s"(carry_out) : "v"(cw), "s"(m));
+    return r;
 }
 
 // a + b, or INF when either is INF (uint64) / saturating at INF (uint32) /
@@ -146,20 +144,77 @@ template <int MODE> struct DpCost<float, MODE> {
     }
 };
 
-// VEC: K == 4, D == 256, 4-byte costs and 16-byte aligned lanes (one 16-byte
-// load per lane and step); otherwise K element loads with a disparity bound.
+// Minimum over the wave, valid in lane 63: row shifts 1, 2, 4, 8 leave
+// each row's minimum in its lane 15, then row_bcast:15 (rows 1 and 3) and
+// row_bcast:31 (rows 2 and 3) fold the rows together. Lanes a DPP read does
+// not reach take the identity (all ones).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t min_dpp(uint32_t v)
+{
+    return min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, CTRL, ROWS, 0xF, false));
+}
+// Stage s (0, 1, 2) of that fold: shifts 1 and 2, shifts 4 and 8, the two
+// broadcasts.
+__device__ __forceinline__ uint32_t wave_min_stage(uint32_t v, int s)
+{
+    if (s == 0) return min_dpp<0x112, 0xF>(min_dpp<0x111, 0xF>(v));
+    if (s == 1) return min_dpp<0x118, 0xF>(min_dpp<0x114, 0xF>(v));
+    return min_dpp<0x143, 0xC>(min_dpp<0x142, 0xA>(v));
+}
+
+// How a lane reads its K cells of a column: LD_ELEM one element load per
+// cell (uint16 volumes with odd K or 2-byte alignment), LD_DW the K cells as
+// consecutive dwords (b128/b96/b64/b32 loads; needs 4-byte aligned cells
+// runs), LD_FULL as LD_DW with D == 64K (no cell of the lane range lies past
+// D, so no disparity mask).
+enum { LD_ELEM = 0, LD_DW = 1, LD_FULL = 2 };
+
+// Cache policy of the cost loads: non-temporal (each cell is read once;
+// measured on a 1920x1080x256 uint32 stream: 6.6 TB/s against 5.7 TB/s
+// with the default policy).
+constexpr int DP_LOAD_NT = 2;
+
+template <typename TIn, int K, int LD>
+__device__ __forceinline__ void dp_load_cells(buf_t r, uint32_t off, TIn (&v)[K])
+{
+    if constexpr (LD == LD_ELEM) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            if constexpr (sizeof(TIn) == 2)
+                v[k] = __builtin_amdgcn_raw_buffer_load_b16(r, off + 2 * k, 0, DP_LOAD_NT);
+            else v[k] = __builtin_bit_cast(TIn, __builtin_amdgcn_raw_buffer_load_b32(r, off + 4 * k, 0, DP_LOAD_NT));
+        }
+    } else {
+        constexpr int NW = K * (int)sizeof(TIn) / 4;
+        uint32_t w[NW];
+        buf_load<NW, 0, DP_LOAD_NT>(r, off, w);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            if constexpr (sizeof(TIn) == 2) v[k] = (uint16_t)(w[k / 2] >> (16 * (k & 1)));
+            else v[k] = __builtin_bit_cast(TIn, w[k]);
+        }
+    }
+}
+
+// Costs are read through buffer resources, one per block of PF columns
+// (based at the block's first column, ranged to the columns of the block
+// that lie in the row): lane l's cells of column x0 + u start at byte
+// l*K*sizeof(TIn) + u*sxb, and columns past the row read 0 from the range
+// check (never consumed), so the prefetch needs neither a clamp nor 64-bit
+// address arithmetic per load. Decision words go through a resource over
+// the row's path words.
 // `only` (fixup launch): rows whose flag is 0 return at once. `flags_out`
 // (bounded launch): receives 1 for each row that breaks the contract.
-template <typename TIn, int K, int PF, bool VEC, int MODE>
+template <typename TIn, int K, int PF, int LD, int MODE>
 __global__ __launch_bounds__(64) void dp_kernel(const TIn *__restrict__ C, int W, int H, int D,
-                                                int64_t sx, int64_t sy, uint32_t P, float Pf,
-                                                int flags, uint8_t *__restrict__ path,
+                                                int64_t sx, int64_t sy, uint32_t P,
+                                                float Pf, int flags, uint8_t *__restrict__ path,
                                                 int16_t *__restrict__ disp, int pitch,
                                                 const uint8_t *__restrict__ only,
                                                 uint8_t *__restrict__ flags_out)
 {
-    static_assert(!VEC || (K == 4 && sizeof(TIn) == 4), "vector loads cover four 4-byte cells");
-    constexpr int CPW = dp_cpw(K), BITS = 32 / CPW;
+    static_assert(LD == LD_ELEM || (K * sizeof(TIn)) % 4 == 0, "dword loads need whole dwords per lane");
+    constexpr int CPW = dp_cpw(K), WORD_BITS = 2 * K * CPW; // decision bits per word
     constexpr bool B32 = MODE == DP_B32;
     typedef DpCost<TIn, MODE> cost_t;
     typedef typename cost_t::type cval_t;
@@ -167,22 +222,24 @@ __global__ __launch_bounds__(64) void dp_kernel(const TIn *__restrict__ C, int W
     const int y = blockIdx.x;
     if (only && only[y] == 0) return; // wave-uniform
     const TIn *crow = C + y * sy;
-    uint32_t *prow = (uint32_t *)(path + (size_t)y * dp_row_bytes(W, K));
+    const uint32_t sxb = (uint32_t)(sx * (int64_t)sizeof(TIn)), cellb = (uint32_t)(D * (int)sizeof(TIn));
+    // resource over columns [x, x + n) of the row (n clipped to the row)
+    auto cols = [&](int x, int n) {
+        const int m = min(n, W - x);
+        return buf_rsrc(crow + x * sx, m > 0 ? (uint32_t)(m - 1) * sxb + cellb : 0u);
+    };
+    const uint32_t prow_bytes = (uint32_t)dp_row_bytes(W, K);
+    const buf_t prs = buf_rsrc(path + (size_t)y * prow_bytes, prow_bytes);
+    const uint32_t lane_off = (uint32_t)(l * K * (int)sizeof(TIn));
 
-    auto load = [&](cval_t(&c)[K], int x) {
-        const int xx = x < W ? x : W - 1;
-        const TIn *src = crow + xx * sx + l * K;
-        if constexpr (VEC) {
-            typedef TIn vec4 __attribute__((ext_vector_type(4)));
-            const vec4 v = *(const vec4 *)src;
-            c[0] = cost_t::get(v.x), c[1] = cost_t::get(v.y), c[2] = cost_t::get(v.z),
-            c[3] = cost_t::get(v.w);
-        } else {
+    auto load = [&](cval_t(&c)[K], buf_t rs, uint32_t off) {
+        TIn v[K];
+        dp_load_cells<TIn, K, LD>(rs, off, v);
 #pragma unroll
-            for (int k = 0; k < K; k++) {
-                if constexpr (MODE == DP_F32) c[k] = l * K + k < D ? cost_t::get(src[k]) : __builtin_inff();
-                else c[k] = l * K + k < D ? cost_t::get(src[k]) : INV32;
-            }
+        for (int k = 0; k < K; k++) {
+            if constexpr (LD == LD_FULL) c[k] = cost_t::get(v[k]);
+            else if constexpr (MODE == DP_F32) c[k] = l * K + k < D ? cost_t::get(v[k]) : __builtin_inff();
+            else c[k] = l * K + k < D ? cost_t::get(v[k]) : INV32;
         }
     };
 
@@ -199,31 +256,37 @@ __global__ __launch_bounds__(64) void dp_kernel(const TIn *__restrict__ C, int W
     }
 
     // bounded contract check: a cell d > x (or d >= D) must hold INV32, any
-    // other cell a cost below 2^22
-    bool bad = false;
-    uint32_t vmax = 0; // running maximum once every d < D is valid
+    // other cell a cost below 2^22. Kept in 32-bit lane values (no per-lane
+    // branches): a violation sets bit 0 of `bad`, and once every d < D is
+    // valid a running maximum of the costs does the same job.
+    uint32_t bad = 0;
+    uint32_t vmax = 0;
     auto check_exact = [&](const cval_t(&c)[K], int x) {
         if constexpr (B32) {
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 const int d = l * K + k;
-                const bool inv = d > x || d >= D;
-                bad |= inv ? (uint32_t)c[k] != INV32 : (uint32_t)c[k] >= DP_COST_LIMIT;
+                const uint32_t inv = ((uint32_t)(x - d) >> 31) | (uint32_t)(d >= D);
+                const uint32_t t = (uint32_t)c[k] ^ (0u - inv); // invalid cells: 0 iff INV32
+                const uint32_t thr = inv ? 1u : DP_COST_LIMIT;
+                bad |= (uint32_t)(t >= thr);
             }
         }
     };
     auto check_max = [&](const cval_t(&c)[K]) {
         if constexpr (B32) {
 #pragma unroll
-            for (int k = 0; k < K; k++)
-                vmax = max(vmax, l * K + k < D ? (uint32_t)c[k] : 0u);
+            for (int k = 0; k < K; k++) {
+                if constexpr (LD == LD_FULL) vmax = max(vmax, (uint32_t)c[k]);
+                else vmax = max(vmax, l * K + k < D ? (uint32_t)c[k] : 0u);
+            }
         }
     };
 
     acc_t A[K];
     {
         cval_t c[K];
-        load(c, 0);
+        load(c, cols(0, 1), lane_off);
         check_exact(c, 0);
 #pragma unroll
         for (int k = 0; k < K; k++) {
@@ -231,73 +294,93 @@ __global__ __launch_bounds__(64) void dp_kernel(const TIn *__restrict__ C, int W
             else A[k] = c[k] == INV32 ? INF : c[k];
         }
     }
-    uint32_t pend = 0; // B32: wave minimum of the last renormalised column
+    // B32: lane minima of the last renormalised column, folded over the
+    // wave across the next three steps (two DPP stages each) and read from
+    // lane 63 at the next renormalisation
+    uint32_t mn = 0;
     cval_t ring[PF][K];
 #pragma unroll
-    for (int u = 0; u < PF; u++) load(ring[u], 1 + u);
+    for (int u = 0; u < PF; u++) load(ring[u], cols(1, PF), lane_off + (uint32_t)u * sxb);
+    // block x0: its refill loads read columns x0 + PF + u through `next`,
+    // its decision words are word (x0 - 1) / CPW + (1 + u) / CPW (a
+    // compile-time immediate per step past st_off)
+    buf_t next;
+    uint32_t st_off = (uint32_t)l * 4;
+    // DPP results keep the invalid value in the lane without a source (lane
+    // 0 for the shift from below, lane 63 for the shift from above), so the
+    // previous result serves as the `old` operand
+    acc_t left = INF, right = INF;
 
-    // one step at column x from ring slot u (which it then refills); the
-    // decision bits go into slot x%CPW of cw, stored when the word is full
-    uint32_t cw = 0;
+    // one step at column x = x0 + u from ring slot u (which it then
+    // refills), x0 = 1 + a multiple of PF (PF % 4 == 0), so x % 4 is the
+    // compile-time (1 + u) % 4 once the block is unrolled. Each cell's
+    // decision is two bits, p = (c1 < c2 && c1 < c3) and q = (c2 < c3)
+    // (move -1 if p, else 0 if q, else +1), shifted into cw in the order
+    // (column, k, p then q), so a word of CPW columns holds its first
+    // column's first p bit at bit 2*K*CPW - 1; stored when the word is full.
+    // Column 0 has no decisions: its fields read (p, q) = (0, 1), "stay".
+    uint32_t cw = 0x55555555u & ((1u << (2 * K)) - 1);
     auto step = [&](int u, int x, bool tail, bool exact) {
+        const int ph = (1 + u) & 3; // == x & 3
         cval_t c[K];
 #pragma unroll
         for (int k = 0; k < K; k++) c[k] = ring[u][k];
-        load(ring[u], x + PF);
         if (exact) check_exact(c, x);
         else check_max(c);
-        const acc_t left = wave_from_prev(A[K - 1]);
-        const acc_t right = wave_from_next(A[0]);
-        uint32_t codes = 0;
+        // S = A + P per cell; the neighbours' S across lanes move by DPP
+        // (the end lanes keep INF, which is INF + P)
+        acc_t S[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) S[k] = dp_add(A[k], Pen);
+        left = wave_from_prev(left, S[K - 1]);
+        right = wave_from_next(right, S[0]);
         acc_t An[K];
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            const acc_t lm = k == 0 ? left : A[k - 1];
-            const acc_t lp = k == K - 1 ? right : A[k + 1];
-            const acc_t c1 = dp_add(lm, Pen);
+            const acc_t c1 = k == 0 ? left : S[k - 1];
             const acc_t c2 = A[k];
-            const acc_t c3 = dp_add(lp, Pen);
-            acc_t mv;
-            uint32_t code;
-            if (c1 < c2 && c1 < c3) {
-                mv = c1;
-                code = 0;
-            } else if (c2 < c3) {
-                mv = c2;
-                code = 1;
-            } else {
-                mv = c3;
-                code = 2;
-            }
+            const acc_t c3 = k == K - 1 ? right : S[k + 1];
+            const bool q = c2 < c3;
+            const acc_t m23 = q ? c2 : c3;
+            const bool p = c1 < m23;
+            const acc_t mv = p ? c1 : m23;
             if constexpr (MODE == DP_F32) An[k] = mv + c[k];
             else if constexpr (B32) An[k] = dp_add(mv, c[k]);
             else An[k] = (c[k] == INV32 || mv == DP_INF) ? DP_INF : mv + c[k];
-            codes |= code << (2 * k);
+            cw = push_bit(push_bit(cw, p), q);
         }
 #pragma unroll
         for (int k = 0; k < K; k++) A[k] = An[k];
         if constexpr (B32) {
-            if ((x & 3) == 3) {
+            if (ph == 3) {
                 // subtract the minimum taken 4 columns ago, then take this
                 // column's (consumed at the next renormalisation)
+                const uint32_t pend = (uint32_t)__builtin_amdgcn_readlane((int)mn, 63);
 #pragma unroll
                 for (int k = 0; k < K; k++) A[k] -= pend;
-                uint32_t mn = A[0];
+                mn = A[0];
 #pragma unroll
                 for (int k = 1; k < K; k++) mn = min(mn, (uint32_t)A[k]);
-                mn = row_min_u32(mn);
-                const uint32_t m0 = (uint32_t)__builtin_amdgcn_readlane((int)mn, 0);
-                const uint32_t m1 = (uint32_t)__builtin_amdgcn_readlane((int)mn, 16);
-                const uint32_t m2 = (uint32_t)__builtin_amdgcn_readlane((int)mn, 32);
-                const uint32_t m3 = (uint32_t)__builtin_amdgcn_readlane((int)mn, 48);
-                pend = min(min(m0, m1), min(m2, m3));
+            } else {
+                mn = wave_min_stage(mn, ph);
             }
         }
-        cw |= codes << (BITS * (x % CPW));
-        if (x % CPW == CPW - 1 || (tail && x == W - 1)) {
-            prow[(size_t)(x / CPW) * 64 + l] = cw;
+        const int slot = ph % CPW; // == x % CPW
+        if (slot == CPW - 1 || (tail && x == W - 1)) {
+            // a partial last word moves up to the full word's positions
+            __builtin_amdgcn_raw_buffer_store_b32(cw << ((CPW - 1 - slot) * 2 * K), prs,
+                                                  st_off + (uint32_t)((1 + u) / CPW) * 256u, 0, 0);
             cw = 0;
         }
+        // refill the slot, PF columns ahead, once its values are consumed
+        // (so the loads reuse its registers instead of a copy of the ring);
+        // the scheduling barriers keep each refill at its step, where the
+        // prefetch distance stays PF through the block, while ALU work of
+        // neighbouring steps may still interleave across them (mask 1: all
+        // non-memory instructions)
+        __builtin_amdgcn_sched_barrier(1);
+        load(ring[u], next, lane_off + (uint32_t)u * sxb);
+        __builtin_amdgcn_sched_barrier(1);
     };
     // whole blocks of PF steps without conditions (so the waits stay per
     // slot; PF % 4 == 0 and x0 % 4 == 1 make every x % CPW and x & 3 a
@@ -305,16 +388,22 @@ __global__ __launch_bounds__(64) void dp_kernel(const TIn *__restrict__ C, int W
     // the bounded contract exactly, the others keep a running maximum.
     static_assert(PF % 4 == 0, "whole decision words and renormalisation periods per block");
     int x0 = 1;
-    for (; x0 + PF <= W && (B32 && x0 < 64 * K); x0 += PF) {
+    constexpr uint32_t st_blk = (uint32_t)(PF / CPW) * 256u;
+    for (; x0 + PF <= W && (B32 && x0 < 64 * K); x0 += PF, st_off += st_blk) {
+        next = cols(x0 + PF, PF);
 #pragma unroll
         for (int u = 0; u < PF; u++) step(u, x0 + u, false, true);
     }
-    for (; x0 + PF <= W; x0 += PF) {
+    for (; x0 + PF <= W; x0 += PF, st_off += st_blk) {
+        next = cols(x0 + PF, PF);
 #pragma unroll
         for (int u = 0; u < PF; u++) step(u, x0 + u, false, false);
     }
+    next = cols(W, 0); // the tail's refills read nothing
     if (x0 == W) { // the last block ended the row inside a word
-        if ((W - 1) % CPW != CPW - 1) prow[(size_t)((W - 1) / CPW) * 64 + l] = cw;
+        if ((W - 1) % CPW != CPW - 1)
+            __builtin_amdgcn_raw_buffer_store_b32(cw << ((CPW - 1 - (W - 1) % CPW) * 2 * K), prs,
+                                                  (uint32_t)((W - 1) / CPW) * 256u + l * 4, 0, 0);
     }
 #pragma unroll
     for (int u = 0; u < PF; u++)
@@ -322,7 +411,7 @@ __global__ __launch_bounds__(64) void dp_kernel(const TIn *__restrict__ C, int W
 
     if constexpr (B32) {
         // the row is exact unless some cost broke the contract
-        const bool row_bad = __builtin_amdgcn_ballot_w64(bad || vmax >= DP_COST_LIMIT) != 0;
+        const bool row_bad = __builtin_amdgcn_ballot_w64(bad != 0 || vmax >= DP_COST_LIMIT) != 0;
         if (l == 0) flags_out[y] = row_bad ? 1 : 0;
         if (row_bad) return; // the fixup launch writes this row
     }
@@ -354,48 +443,64 @@ __global__ __launch_bounds__(64) void dp_kernel(const TIn *__restrict__ C, int W
     }
 
     // backtrack over blocks of 64 columns (64 / CPW decision words per
-    // lane): each lane loads its block's words with independent loads, then
-    // the whole wave walks the columns in lockstep with a wave-uniform
-    // disparity z, taking column x's decision bits from lane z / K by
-    // readlane; lane j keeps the disparity of the block's column j and
-    // stores it once
+    // lane): each lane loads its block's words with independent loads and
+    // turns every (p, q) field into the move + 1 (p -> 0, q -> 1, else 2);
+    // then the whole wave walks the columns in lockstep with a wave-uniform
+    // disparity z, taking column x's field from lane z / K by readlane, and
+    // lane j of the block keeps column j's disparity. A -1
+    // move never starts at d = 0 (its c1 is the invalid value, never below
+    // c2 and c3) and a +1 move from d = D - 1 stays at D - 1, so the walk
+    // needs only an upper clamp.
     constexpr int WPB = 64 / CPW; // decision words per block and lane
     const bool refbt = flags & TDVH_DP_REF_BACKTRACK;
-    int z = __builtin_amdgcn_readfirstlane(start);
+    uint32_t z = (uint32_t)__builtin_amdgcn_readfirstlane(start);
     int16_t *out = disp + (size_t)y * pitch;
-    const int nw = (W + CPW - 1) / CPW; // decision words per lane
-    for (int wb = ((W - 1) / CPW) & ~(WPB - 1); wb >= 0; wb -= WPB) {
-        uint32_t wv[WPB];
+    // the next block's words are fetched before the current block is walked
+    auto fetch = [&](int wb, uint32_t(&wv)[WPB]) {
 #pragma unroll
-        for (int i = 0; i < WPB; i++) wv[i] = wb + i < nw ? prow[(size_t)(wb + i) * 64 + l] : 0u;
-        int mine = 0;
+        for (int i = 0; i < WPB; i++) // words past the row read 0
+            wv[i] = __builtin_amdgcn_raw_buffer_load_b32(prs, (uint32_t)(wb + i) * 256u + l * 4, 0, 0);
+    };
+    auto walk = [&](int wb, uint32_t(&wv)[WPB], bool partial) {
+#pragma unroll
+        for (int i = 0; i < WPB; i++) {
+            const uint32_t w = wv[i];
+            const uint32_t np = ~(w >> 1) & 0x55555555u; // fields with p = 0, at their low bit
+            wv[i] = (np & w) | ((np & ~w) << 1);
+        }
+        uint32_t mine = 0;
 #pragma unroll
         for (int i = WPB - 1; i >= 0; i--) {
 #pragma unroll
             for (int b = CPW - 1; b >= 0; b--) {
-                const int x = (wb + i) * CPW + b;
-                if (x > W - 1) continue; // wave-uniform
-                const uint32_t word = (uint32_t)__builtin_amdgcn_readlane((int)wv[i], z / K);
-                // x = 0 has no stored decision: "stay"
-                const int code = x == 0 ? 1 : (int)((word >> (BITS * b + 2 * (z % K))) & 3u);
-                const int nz = z + code - 1;
-                if (refbt) {
-                    if (nz >= 0 && nz < D) z = nz;
-                    mine = l == CPW * i + b ? z : mine;
-                } else {
-                    mine = l == CPW * i + b ? z : mine;
-                    if (x > 0 && nz >= 0 && nz < D) z = nz;
-                }
+                if (partial && (wb + i) * CPW + b > W - 1) continue; // wave-uniform
+                const uint32_t word = (uint32_t)__builtin_amdgcn_readlane((int)wv[i], (int)(z / K));
+                const uint32_t m = (word >> (WORD_BITS - 2 - 2 * (K * b + (int)(z % K)))) & 3u;
+                const uint32_t nz = min(z + m - 1, (uint32_t)(D - 1));
+                if (refbt) z = nz;
+                mine = l == CPW * i + b ? z : mine;
+                if (!refbt) z = nz;
             }
         }
         if (CPW * wb + l < W) out[CPW * wb + l] = (int16_t)mine;
+    };
+    uint32_t cur[WPB], nxt[WPB];
+    int wb = ((W - 1) / CPW) & ~(WPB - 1);
+    fetch(wb, cur);
+    if (wb >= WPB) fetch(wb - WPB, nxt);
+    walk(wb, cur, true); // the last block may end inside it
+    for (wb -= WPB; wb >= 0; wb -= WPB) {
+#pragma unroll
+        for (int i = 0; i < WPB; i++) cur[i] = nxt[i];
+        if (wb >= WPB) fetch(wb - WPB, nxt);
+        walk(wb, cur, false);
     }
 }
 
 struct DpLaunch {
     const void *C;
     int W, H, D;
-    int64_t sx, sy;
+    int64_t sx, sy; // strides in elements
     uint32_t P;
     float Pf;
     int flags;
@@ -404,50 +509,69 @@ struct DpLaunch {
     int pitch;
     const uint8_t *only;
     uint8_t *flags_out;
+    int ld; // LD_ELEM, LD_DW or LD_FULL
 };
 
-template <typename TIn, int MODE>
-static void launch_dp(const DpLaunch &a, bool vec, hipStream_t s)
+// Prefetch depth (columns of costs in flight per row): 16 for K <= 4, 8
+// above (the ring then holds 48 or 64 cost registers).
+template <typename TIn, int K, int LD, int MODE>
+static void launch_dp_k(const DpLaunch &a, hipStream_t s)
 {
-    const TIn *cp = (const TIn *)a.C;
-#define DP_ARGS cp, a.W, a.H, a.D, a.sx, a.sy, a.P, a.Pf, a.flags, a.path, a.disp, a.pitch, a.only, a.flags_out
-    switch ((a.D + 63) / 64) {
-    case 1: dp_kernel<TIn, 1, 16, false, MODE><<<a.H, 64, 0, s>>>(DP_ARGS); break;
-    case 2: dp_kernel<TIn, 2, 16, false, MODE><<<a.H, 64, 0, s>>>(DP_ARGS); break;
-    case 3: dp_kernel<TIn, 3, 16, false, MODE><<<a.H, 64, 0, s>>>(DP_ARGS); break;
-    case 4:
-        if constexpr (sizeof(TIn) == 4) {
-            if (vec) {
-                dp_kernel<TIn, 4, 16, true, MODE><<<a.H, 64, 0, s>>>(DP_ARGS);
-                break;
-            }
-        }
-        dp_kernel<TIn, 4, 16, false, MODE><<<a.H, 64, 0, s>>>(DP_ARGS);
-        break;
-    case 5: case 6: dp_kernel<TIn, 6, 8, false, MODE><<<a.H, 64, 0, s>>>(DP_ARGS); break;
-    default: dp_kernel<TIn, 8, 8, false, MODE><<<a.H, 64, 0, s>>>(DP_ARGS); break;
+    constexpr int PF = K > 4 ? 8 : 16;
+    dp_kernel<TIn, K, PF, LD, MODE><<<a.H, 64, 0, s>>>(
+        (const TIn *)a.C, a.W, a.H, a.D, a.sx, a.sy, a.P, a.Pf, a.flags, a.path, a.disp,
+        a.pitch, a.only, a.flags_out);
+}
+
+template <typename TIn, int K, int MODE>
+static void launch_dp_ld(const DpLaunch &a, hipStream_t s)
+{
+    if constexpr ((K * sizeof(TIn)) % 4 != 0) {
+        launch_dp_k<TIn, K, LD_ELEM, MODE>(a, s); // odd K of uint16 cells
+    } else {
+        if (a.ld == LD_FULL) launch_dp_k<TIn, K, LD_FULL, MODE>(a, s);
+        else if (a.ld == LD_DW || sizeof(TIn) == 4) launch_dp_k<TIn, K, LD_DW, MODE>(a, s);
+        else if constexpr (sizeof(TIn) == 2) launch_dp_k<TIn, K, LD_ELEM, MODE>(a, s);
     }
-#undef DP_ARGS
+}
+
+// K = disparities per lane: 1..4, then 6 (D <= 384) and 8 (D <= 512).
+__host__ __device__ constexpr int dp_k(int D)
+{
+    return (D + 63) / 64 <= 4 ? (D + 63) / 64 : ((D + 63) / 64 <= 6 ? 6 : 8);
+}
+
+template <typename TIn, int MODE>
+static void launch_dp(const DpLaunch &a, hipStream_t s)
+{
+    switch (dp_k(a.D)) {
+    case 1: launch_dp_ld<TIn, 1, MODE>(a, s); break;
+    case 2: launch_dp_ld<TIn, 2, MODE>(a, s); break;
+    case 3: launch_dp_ld<TIn, 3, MODE>(a, s); break;
+    case 4: launch_dp_ld<TIn, 4, MODE>(a, s); break;
+    case 6: launch_dp_ld<TIn, 6, MODE>(a, s); break;
+    default: launch_dp_ld<TIn, 8, MODE>(a, s); break;
+    }
 }
 
 // Bounded uint32 launch, then the uint64 recomputation of the rows it
 // flagged (a launch whose waves return at once when no row is flagged).
 template <typename TIn>
-static int run_dp_integer(const DpLaunch &a, bool bounded, bool vec, hipStream_t s)
+static int run_dp_integer(const DpLaunch &a, bool bounded, hipStream_t s)
 {
     if (!bounded) {
         prof_begin(s);
-        launch_dp<TIn, DP_U64>(a, vec, s);
+        launch_dp<TIn, DP_U64>(a, s);
         return check_launch("dp_kernel", s);
     }
     prof_begin(s);
-    launch_dp<TIn, DP_B32>(a, vec, s);
+    launch_dp<TIn, DP_B32>(a, s);
     TDVH_TRY(check_launch("dp_kernel", s));
     DpLaunch f = a;
     f.only = a.flags_out;
     f.flags_out = nullptr;
     prof_begin(s);
-    launch_dp<TIn, DP_U64>(f, vec, s);
+    launch_dp<TIn, DP_U64>(f, s);
     return check_launch("dp_fixup_kernel", s);
 }
 
@@ -488,17 +612,27 @@ static int dp_run(tdvh_ctx *ctx, const tdvh_dsi *C, uint32_t p_occ, float pf, in
                      0,
                  "bad flags 0x%x", flags);
     hipStream_t s = (hipStream_t)stream;
-    const int K = (D + 63) / 64 <= 4 ? (D + 63) / 64 : ((D + 63) / 64 <= 6 ? 6 : 8);
+    const int K = dp_k(D);
+    const int64_t esz = C->dtype == TDVH_U16 ? 2 : 4;
+    // blocks of up to 16 columns are read through 32-bit range-checked
+    // byte offsets
+    TDVH_REQUIRE(C->stride_x >= D && C->stride_y >= 0, "bad strides %lld, %lld",
+                 (long long)C->stride_x, (long long)C->stride_y);
+    TDVH_REQUIRE(16 * C->stride_x * esz + D * esz < (1ll << 31) &&
+                     dp_row_bytes(W, K) < (1ull << 31),
+                 "column stride %lld or width %d too large", (long long)C->stride_x, W);
+    TDVH_REQUIRE(((uintptr_t)C->ptr & (esz - 1)) == 0, "volume not aligned to its cells");
     void *pb = nullptr;
     TDVH_TRY(ctx->scratch("dp_path", dp_row_bytes(W, K) * H, &pb));
-    DpLaunch a{C->ptr, W, H, D, C->stride_x, C->stride_y, p_occ, pf, flags,
-               (uint8_t *)pb, disp, disp_pitch, nullptr, nullptr};
-    const size_t esz = C->dtype == TDVH_U16 ? 2 : 4;
-    const bool vec = D == 256 && esz == 4 && C->stride_x % 4 == 0 && C->stride_y % 4 == 0 &&
-                     ((uintptr_t)C->ptr & 15) == 0;
+    // dword loads where every lane's cells start on a 4-byte boundary
+    const bool dw = esz == 4 || (((uintptr_t)C->ptr & 3) == 0 && C->stride_x % 2 == 0 &&
+                                 C->stride_y % 2 == 0 && K % 2 == 0);
+    DpLaunch a{C->ptr, W, H, D, C->stride_x, C->stride_y,
+               p_occ, pf, flags, (uint8_t *)pb, disp, disp_pitch, nullptr, nullptr,
+               !dw ? LD_ELEM : (D == 64 * K ? LD_FULL : LD_DW)};
     if (C->dtype == TDVH_F32) {
         prof_begin(s);
-        launch_dp<float, DP_F32>(a, vec, s);
+        launch_dp<float, DP_F32>(a, s);
         return check_launch("dp_kernel", s);
     }
     // 32-bit sums for every uint16 volume and for uint32 volumes the caller
@@ -510,8 +644,8 @@ static int dp_run(tdvh_ctx *ctx, const tdvh_dsi *C, uint32_t p_occ, float pf, in
         TDVH_TRY(ctx->scratch("dp_row_flags", (size_t)H, &fb));
         a.flags_out = (uint8_t *)fb;
     }
-    if (C->dtype == TDVH_U16) return run_dp_integer<uint16_t>(a, bounded, vec, s);
-    return run_dp_integer<uint32_t>(a, bounded, vec, s);
+    if (C->dtype == TDVH_U16) return run_dp_integer<uint16_t>(a, bounded, s);
+    return run_dp_integer<uint32_t>(a, bounded, s);
 }
 
 extern "C" int tdvh_dp(tdvh_ctx *ctx, const tdvh_dsi *C, uint32_t p_occ, int flags, int16_t *disp,

@@ -332,36 +332,7 @@ __device__ __forceinline__ void sgm_line_group(const SgmArgs &a, int dir, int li
 // for x' < 0, which give C >= 0xFE01, a cost above every valid path cost.
 // Needs every plane below 2^31 bytes (buf_lines_fit).
 
-typedef __amdgpu_buffer_rsrc_t buf_t;
-typedef unsigned int tdvh_u32x2 __attribute__((ext_vector_type(2)));
-typedef unsigned int tdvh_u32x3 __attribute__((ext_vector_type(3)));
-typedef unsigned int tdvh_u32x4 __attribute__((ext_vector_type(4)));
 constexpr uint32_t BUF_OOB = 0x80000000u;
-
-__device__ __forceinline__ buf_t buf_rsrc(const void *base, uint32_t bytes)
-{
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes,
-                                             0x00020000);
-}
-
-// N consecutive dwords at byte offset off (0 outside the buffer).
-template <int N, int I = 0>
-__device__ __forceinline__ void buf_load(buf_t r, uint32_t off, uint32_t *dst)
-{
-    if constexpr (N - I >= 4) {
-        const tdvh_u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off + 4 * I, 0, 0);
-        dst[I] = v.x, dst[I + 1] = v.y, dst[I + 2] = v.z, dst[I + 3] = v.w;
-        buf_load<N, I + 4>(r, off, dst);
-    } else if constexpr (N - I == 3) {
-        const tdvh_u32x3 v = __builtin_amdgcn_raw_buffer_load_b96(r, off + 4 * I, 0, 0);
-        dst[I] = v.x, dst[I + 1] = v.y, dst[I + 2] = v.z;
-    } else if constexpr (N - I == 2) {
-        const tdvh_u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, off + 4 * I, 0, 0);
-        dst[I] = v.x, dst[I + 1] = v.y;
-    } else if constexpr (N - I == 1) {
-        dst[I] = __builtin_amdgcn_raw_buffer_load_b32(r, off + 4 * I, 0, 0);
-    }
-}
 
 // Cache-policy operand of the plane stores: non-temporal (the planes are
 // read once, by a later kernel; measured: the sum over them runs ~10 %
