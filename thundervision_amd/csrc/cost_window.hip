@@ -86,9 +86,8 @@ __device__ __forceinline__ uint32_t wave_min16(uint32_t (&key)[16], int lane)
     return min(k, (uint32_t)__builtin_amdgcn_ds_bpermute((lane ^ 32) << 2, (int)k));
 }
 
-// Two waves per SIMD: the other modes fit in 256 registers as they are; the
-// fused NCC + WTA then spills ~160 bytes per lane, and runs 0.85 ms at
-// 1920x1080 D = 256 against 0.95 ms at one wave per SIMD without spills.
+// At most 256 registers (two waves per SIMD or more): every mode fits
+// without spills at 9x9 windows and D = 256.
 template <int RAD, int NCH, int MODE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void window_kernel(const WinArgs a)
 {
@@ -118,9 +117,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void wi
         const int vy = v < 0 ? 0 : (v >= H ? H - 1 : v);
         const uint8_t *lr = a.L + (size_t)vy * a.pitch;
         const uint8_t *rr = a.R + (size_t)vy * a.pitch;
+        // fused NCC + WTA: the clamped columns are recomputed per row (hidden
+        // from the compiler, which would otherwise keep them all as 64-bit
+        // offsets across the row loop and spill them)
+        int ln = lane;
+        if constexpr (MODE == WIN_NCC_WTA) asm volatile("" : "+v"(ln));
 #pragma unroll
         for (int q = 0; q < PER_LANE; q++) {
-            const int k = lane + 64 * q;
+            const int k = ln + 64 * q;
             uint32_t val = 0;
             if (k < NC) {
                 const int u = lbase + k;
@@ -167,14 +171,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void wi
         if constexpr (NCC) {
             if (v - RAD >= y0) {
                 const size_t pl = (size_t)W * H, row = (size_t)(v - RAD) * W;
+                int ln = lane; // as in fetch
+                if constexpr (MODE == WIN_NCC_WTA) asm volatile("" : "+v"(ln));
                 if (lane < WIN_TX) {
-                    const size_t o = row + min(x0 + lane, W - 1);
+                    const size_t o = row + min(x0 + ln, W - 1);
                     lpre = a.box[o];
                     lqpre = (uint32_t)a.box[pl + o];
                 }
 #pragma unroll
                 for (int q = 0; q < B_PER_LANE; q++) {
-                    const int k = lane + 64 * q;
+                    const int k = ln + 64 * q;
                     const int u = x0 - a.d0 - (NCH * 64 - 1) + k; // only 0 <= u < W are read later
                     const size_t o = row + (u < 0 ? 0 : (u >= W ? W - 1 : u));
                     bpre[q] = a.box[2 * pl + o];
@@ -186,29 +192,45 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void wi
         }
         const bool leave = v - vfirst >= NWIN;
         const int sout = (v - vfirst - NWIN) % SLOTS;
-        // the entering (and leaving) row's left values, then per chunk the
-        // lane's right values, read as whole rows before the arithmetic
-        int lin[NC], lout[NC];
+        // per group of CG columns: the entering (and leaving) row's left
+        // values, then per chunk the lane's right values, read before the
+        // arithmetic. Groups of 8 keep the staged values within a few
+        // registers beside the column sums (the fused NCC + WTA needs that
+        // to fit 256 registers; measured at 1920x1080 D = 256, SSD runs
+        // faster with groups too, SSD + WTA and the NCC volume with whole
+        // rows)
+        constexpr int CG = (MODE == WIN_SSD || MODE == WIN_NCC_WTA) ? 8 : NC;
 #pragma unroll
-        for (int i = 0; i < NC; i++) lin[i] = (int)ring[sin][i];
-        if (leave) {
+        for (int g = 0; g < NC; g += CG) {
+            const int n = NC - g < CG ? NC - g : CG;
+            int lin[CG], lout[CG];
 #pragma unroll
-            for (int i = 0; i < NC; i++) lout[i] = (int)ring[sout][i];
-        }
-#pragma unroll
-        for (int c = 0; c < NCH; c++) {
-            const int rb = NC + NCH * 64 - 1 - (c * 64 + lane); // right value of column 0
-            int rin[NC];
-#pragma unroll
-            for (int i = 0; i < NC; i++) rin[i] = (int)ring[sin][rb + i];
-#pragma unroll
-            for (int i = 0; i < NC; i++) col[c][i] += term(lin[i], rin[i]);
+            for (int i = 0; i < CG; i++)
+                if (i < n) lin[i] = (int)ring[sin][g + i];
             if (leave) {
-                int rout[NC];
 #pragma unroll
-                for (int i = 0; i < NC; i++) rout[i] = (int)ring[sout][rb + i];
+                for (int i = 0; i < CG; i++)
+                    if (i < n) lout[i] = (int)ring[sout][g + i];
+            }
 #pragma unroll
-                for (int i = 0; i < NC; i++) col[c][i] -= term(lout[i], rout[i]);
+            for (int c = 0; c < NCH; c++) {
+                const int rb = NC + NCH * 64 - 1 - (c * 64 + lane) + g; // right value of column g
+                int rin[CG];
+#pragma unroll
+                for (int i = 0; i < CG; i++)
+                    if (i < n) rin[i] = (int)ring[sin][rb + i];
+#pragma unroll
+                for (int i = 0; i < CG; i++)
+                    if (i < n) col[c][g + i] += term(lin[i], rin[i]);
+                if (leave) {
+                    int rout[CG];
+#pragma unroll
+                    for (int i = 0; i < CG; i++)
+                        if (i < n) rout[i] = (int)ring[sout][rb + i];
+#pragma unroll
+                    for (int i = 0; i < CG; i++)
+                        if (i < n) col[c][g + i] -= term(lout[i], rout[i]);
+                }
             }
         }
         const int y = v - RAD; // output row whose window is complete
@@ -313,7 +335,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void wi
                 key[t] = key[t] == m ? a.d0 + ((bcs >> (2 * t)) & 3u) * 64 + (uint32_t)lane : 0xFFFFu;
             }
             const uint32_t dmin = wave_min16(key, lane);
-            const int t = (lane >> 1) & 15;
+            int t = (lane >> 1) & 15;
+            asm volatile("" : "+v"(t)); // no row-invariant 64-bit address kept
             if ((lane & 33) == 0 && x0 + t < W)
                 ((int16_t *)a.out)[(size_t)y * a.disp_pitch + x0 + t] = (int16_t)dmin;
         } else if constexpr (MODE == WIN_NCC) {
