@@ -403,6 +403,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--profile-run", action="store_true")
+    ap.add_argument("--window-only", action="store_true",
+                    help="run only the 1920x1080 D=256 window-cost / DP / WTA paths (for profilers)")
     ap.add_argument("--copy-calibration", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -451,6 +453,9 @@ def main():
         return (time.perf_counter() - t0) / steps
 
     cells = W * H * D
+    if args.window_only:
+        print(json.dumps({"window_1080p": bench_window(tv, synth, torch, ctx)}))
+        return
     if args.profile_run:
         for _ in range(args.steps):
             step()
