@@ -206,3 +206,47 @@ def test_disp_to_unit_float_is_the_ieee_quotient(tv, dev, D):
     got = tv.disp_to_unit_float(to_dev(d, dev), D).cpu().numpy()
     want = d.astype(np.float32) / np.float32(D)  # one IEEE single division
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+# ---- DP: every register shape and load form ----------------------------------------
+
+@pytest.mark.parametrize("D", [20, 64, 100, 128, 150, 192, 256, 300, 384, 450, 512])
+@pytest.mark.parametrize("kind", ["u16", "u16_unaligned", "u32", "u32_bounded", "u32_xyd", "f32"])
+def test_dp_every_register_shape(tv, dev, D, kind):
+    """Every disparities-per-lane shape (K = 1..4, 6, 8), full and partial
+    lane ranges, dword loads and element loads (uint16 volumes with odd K
+    or 2-byte aligned cells), the reference's [x][y][d] strides, for all
+    three cost types, against the oracle."""
+    rng = np.random.default_rng(D * 7 + len(kind))
+    H, W = 5, 70 + D // 3
+    x = np.arange(W)[None, :, None]
+    d = np.arange(D)[None, None, :]
+    inv = np.broadcast_to(d > x, (H, W, D))
+    if kind.startswith("u16"):
+        C = rng.integers(0, 600, (H, W, D)).astype(np.uint16)
+        C[inv] = oracle.INV16
+        C[rng.random(C.shape) < 0.002] = oracle.INV16  # some rows leave the 32-bit contract
+        want = oracle.dp_u16(C, 20)
+        t = to_dev(C.view(np.int16), dev)
+        if kind == "u16_unaligned":  # cells at odd 2-byte offsets: element loads
+            big = torch.zeros((H, W, D + 1), dtype=torch.int16, device=dev)
+            big[:, :, 1:] = t
+            t = big[:, :, 1:]
+        got = tv.dp(t, 20)
+    elif kind.startswith("u32"):
+        hi = (1 << 31) if kind == "u32" else (1 << 22)
+        C = rng.integers(0, hi, (H, W, D), dtype=np.uint64).astype(np.uint32)
+        C[inv] = 0xFFFFFFFF
+        want = oracle.dp(C, 20)
+        if kind == "u32_xyd":
+            t = to_dev(np.ascontiguousarray(C.transpose(1, 0, 2)).view(np.int32), dev)
+            t = t.permute(1, 0, 2)
+        else:
+            t = to_dev(C.view(np.int32), dev)
+        got = tv.dp(t, 20, 0 if kind == "u32" else tv.DP_COST_BOUNDED)
+    else:
+        C = rng.random((H, W, D)).astype(np.float32)
+        C[inv] = np.inf
+        want = oracle.dp_f32(C, np.float32(0.125))
+        got = tv.dp(to_dev(C, dev), 0.125)
+    assert (got.cpu().numpy() == want).all()
