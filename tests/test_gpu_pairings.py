@@ -42,6 +42,21 @@ def test_sgm_over_ssd_volume(tv, dev, W, H, D, r, mask):
     assert (disp.cpu().numpy() == oracle.wta(want_S)).all()
 
 
+@pytest.mark.parametrize("W,H,D,p2", [(50, 15, 150, 32), (23, 61, 190, 255), (64, 9, 64, 256)])
+def test_sgm_u32_reference_strides(tv, dev, W, H, D, p2):
+    """uint32 volumes in the reference's [x][y][d] order (3 disparities per
+    lane; P2 at and just above the byte planes' limit)."""
+    rng = np.random.default_rng(W * H + D)
+    C = rng.integers(0, 1 << 21, (H, W, D)).astype(np.uint32)
+    x = np.arange(W)[None, :, None]
+    C[np.broadcast_to(np.arange(D)[None, None, :] > x, C.shape)] = oracle.INV32
+    Cx = to_dev(np.ascontiguousarray(C.transpose(1, 0, 2)).view(np.int32), dev).permute(1, 0, 2)
+    disp, S = tv.sgm(Cx, p1=7, p2=p2, return_S=True)
+    want = oracle.sgm_u32(C, 7, p2)
+    assert (S.cpu().numpy().view(np.uint32) == want).all()
+    assert (disp.cpu().numpy() == oracle.wta(want)).all()
+
+
 def test_sgm_u32_invalid_holes_restarts_and_saturation(tv, dev):
     rng = np.random.default_rng(21)
     H, W, D = 17, 29, 70

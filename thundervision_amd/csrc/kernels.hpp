@@ -23,7 +23,8 @@ int run_sgm_ref(tdvh_ctx *ctx, const uint32_t *C_split, int W, int H, int D, int
 
 // SGM over a strided uint32 or float volume into S (same dtype, strided),
 // directions of p->dir_mask in increasing index (sgm_wide.hip).
-int run_sgm_wide(const tdvh_dsi *C, const tdvh_sgm_params *p, const tdvh_dsi *S, hipStream_t s);
+int run_sgm_wide(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params *p, const tdvh_dsi *S,
+                 hipStream_t s);
 
 int popcount8(int mask);
 

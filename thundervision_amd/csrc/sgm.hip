@@ -386,7 +386,7 @@ static int sgm_wide(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params *par
         TDVH_TRY(ctx->scratch("sgm_wide_s", (size_t)W * H * D * 4, &sb));
         TDVH_TRY(tdvh_dsi_init(&S, sb, W, H, D, C->dtype, TDVH_LAYOUT_STRIDED));
     }
-    TDVH_TRY(run_sgm_wide(C, params, &S, (hipStream_t)stream));
+    TDVH_TRY(run_sgm_wide(ctx, C, params, &S, (hipStream_t)stream));
     return tdvh_wta(ctx, &S, TDVH_WTA_LOWEST, disp, disp_pitch, stream);
 }
 

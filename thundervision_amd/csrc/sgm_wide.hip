@@ -19,9 +19,18 @@
 //
 // Mapping: one wave per line; lane l holds d = l*K + k (K = ceil(D/64) <= 4)
 // in registers; the d-1 / d+1 neighbours across lanes move with whole-wave
-// DPP shifts (the vacated end lane reads the invalid value); m is a DPP row
-// minimum and four readlanes. Costs and the running sum stream through a
-// prefetch ring of PF steps.
+// DPP shifts (the vacated end lane reads the invalid value); m is a DPP
+// minimum over the wave. Costs stream through a prefetch ring of PF steps.
+//
+// uint32 volumes with P2 <= 255 (byte planes): since E = L - C lies in
+// [0, P2] (0 at a path start or restart), every line of every enabled
+// direction runs in one launch and stores E as one byte per cell in its
+// direction's plane; a second kernel forms S = n*C + sum of the n planes'
+// E, saturating at the invalid value (a sum of non-negative terms
+// saturates to the same value in any order, and L = C + E saturates only
+// when S does). That moves 5 bytes per cell and direction through the
+// scans instead of 12 and fills the GPU with every direction's lines at
+// once. Other volumes run one launch per direction adding into S.
 #include <type_traits>
 
 #include "sgm_device.hpp"
@@ -181,6 +190,250 @@ static void launch_wide_k(const WideArgs<T> &a, int dir, bool first, hipStream_t
     else sgm_wide_dir_kernel<T, K, PF, false><<<blocks, 256, 0, s>>>(a, dir);
 }
 
+// ---- uint32 volumes, byte planes ------------------------------------------------
+
+// Minimum over the wave, returned to every lane: row shifts 1, 2, 4, 8 leave
+// each row's minimum in its lane 15, row_bcast:15 / :31 fold the rows into
+// lane 63.
+template <int CTRL, int ROWS> __device__ __forceinline__ uint32_t min_dpp_wide(uint32_t v)
+{
+    return min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, CTRL, ROWS, 0xF, false));
+}
+__device__ __forceinline__ uint32_t wave_min_u32_all(uint32_t v)
+{
+    v = min_dpp_wide<0x111, 0xF>(v);
+    v = min_dpp_wide<0x112, 0xF>(v);
+    v = min_dpp_wide<0x114, 0xF>(v);
+    v = min_dpp_wide<0x118, 0xF>(v);
+    v = min_dpp_wide<0x142, 0xA>(v);
+    v = min_dpp_wide<0x143, 0xC>(v);
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+struct WideEArgs {
+    const uint8_t *C;   // uint32 cost volume
+    int64_t csx, csy;   // its strides in bytes
+    uint8_t *planes;    // one W*H*D byte plane per launched direction, [y][x][d]
+    size_t plane_bytes;
+    int W, H, D;
+    uint32_t P1, P2;
+    int ndirs;
+    int dir[8];
+    int begin[9]; // first wave of each direction's lines (the last: all waves)
+};
+
+// One wave per line, the lines of every launched direction in one grid
+// (longest directions first). Costs of step k are read through a buffer
+// resource based at the step's pixel (ranged to its D cells, or to nothing
+// past the line), so the lane offsets stay constant and the prefetch needs
+// no clamp.
+template <int K, int PF>
+__global__ __launch_bounds__(64) void sgm_wide_e_kernel(const WideEArgs a)
+{
+    const int gw = blockIdx.x;
+    int s = 0;
+    while (s < a.ndirs - 1 && gw >= a.begin[s + 1]) s++; // wave-uniform
+    const int dir = a.dir[s], line = gw - a.begin[s];
+    int sx, sy, len;
+    line_geom(dir, line, a.W, a.H, sx, sy, len);
+    const int dx = dir_dx(dir), dy = dir_dy(dir);
+    const int l = threadIdx.x, d0 = l * K;
+    const uint32_t lane_off = (uint32_t)d0 * 4, cellb = (uint32_t)a.D * 4;
+    const int64_t stepb = dx * a.csx + dy * a.csy;
+    const uint8_t *fp = a.C + sx * a.csx + sy * a.csy;
+    int kf = 0;
+    auto fetch = [&](uint32_t(&slot)[K]) {
+        const buf_t r = buf_rsrc(fp, kf < len ? cellb : 0u);
+        buf_load<K, 0, 2>(r, lane_off, slot); // read once: non-temporal
+        fp += stepb;
+        kf++;
+    };
+    uint32_t ring[PF][K];
+#pragma unroll
+    for (int u = 0; u < PF; u++) fetch(ring[u]);
+
+    const buf_t pr = buf_rsrc(a.planes + (size_t)s * a.plane_bytes, (uint32_t)a.plane_bytes);
+    uint32_t eo = (uint32_t)(((size_t)sy * a.W + sx) * a.D) + (uint32_t)d0;
+    const uint32_t de = (uint32_t)((dy * a.W + dx) * a.D);
+    const int nvalid = a.D - d0; // cells of this lane below D (may be <= 0)
+
+    uint32_t Lp[K];
+#pragma unroll
+    for (int q = 0; q < K; q++) Lp[q] = INV32;
+    uint32_t m = INV32; // every cell of L' invalid: the next step starts the path
+
+    auto body = [&](uint32_t(&slot)[K]) {
+        uint32_t c[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) c[q] = q < nvalid ? slot[q] : INV32;
+        fetch(slot);
+        uint32_t E[K], Ln[K];
+        if (m == INV32) { // wave-uniform: path start or restart
+#pragma unroll
+            for (int q = 0; q < K; q++) E[q] = 0;
+        } else {
+            const uint32_t left = wide_dpp<0x138>(Lp[K - 1]); // wave_shr:1
+            const uint32_t right = wide_dpp<0x130>(Lp[0]);    // wave_shl:1
+#pragma unroll
+            for (int q = 0; q < K; q++) {
+                const uint32_t lm = q == 0 ? left : Lp[q - 1];
+                const uint32_t lp = q == K - 1 ? right : Lp[q + 1];
+                const uint32_t nb = lm < lp ? lm : lp;
+                const uint32_t up = __builtin_elementwise_add_sat(nb, a.P1);
+                const uint32_t t = Lp[q] < up ? Lp[q] : up;
+                E[q] = (t - m) < a.P2 ? (t - m) : a.P2;
+            }
+        }
+        uint32_t mn = INV32, e = 0;
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            Ln[q] = __builtin_elementwise_add_sat(c[q], E[q]);
+            Lp[q] = Ln[q];
+            mn = min(mn, Ln[q]);
+            e |= E[q] << (8 * q);
+        }
+        m = wave_min_u32_all(mn);
+        if (nvalid >= K) {
+            if constexpr (K == 4) __builtin_amdgcn_raw_buffer_store_b32(e, pr, eo, 0, 0);
+            else if constexpr (K == 2) __builtin_amdgcn_raw_buffer_store_b16((unsigned short)e, pr, eo, 0, 0);
+            else if constexpr (K == 1) __builtin_amdgcn_raw_buffer_store_b8((unsigned char)e, pr, eo, 0, 0);
+            else {
+                __builtin_amdgcn_raw_buffer_store_b16((unsigned short)e, pr, eo, 0, 0);
+                __builtin_amdgcn_raw_buffer_store_b8((unsigned char)(e >> 16), pr, eo + 2, 0, 0);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < K; q++)
+                if (q < nvalid)
+                    __builtin_amdgcn_raw_buffer_store_b8((unsigned char)(e >> (8 * q)), pr, eo + q, 0, 0);
+        }
+        eo += de;
+    };
+    int k0 = 0;
+    for (; k0 + PF <= len; k0 += PF) {
+#pragma unroll
+        for (int u = 0; u < PF; u++) body(ring[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < PF; u++)
+        if (k0 + u < len) body(ring[u]);
+}
+
+// S(x, y, d) = n*C + sum of the n planes' E, saturating at the invalid value
+// (invalid where C is); one thread per pixel and 4 consecutive d. VEC: D % 4
+// == 0 and 16-byte aligned cost and sum cells (one 16-byte load of C, one
+// 4-byte load per plane, one 16-byte store of S).
+__device__ __forceinline__ uint32_t wide_s(uint32_t c, uint32_t esum, int n)
+{
+    const uint64_t v = (uint64_t)c * (uint64_t)n + esum;
+    return c == INV32 ? INV32 : (uint32_t)(v < (uint64_t)INV32 ? v : (uint64_t)INV32);
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(256) void sgm_wide_sum_kernel(const uint8_t *__restrict__ C, int64_t csx,
+                                                           int64_t csy, const uint8_t *__restrict__ planes,
+                                                           size_t plane_bytes, int n, uint8_t *__restrict__ S,
+                                                           int64_t ssx, int64_t ssy, int W, int H, int D)
+{
+    const int G = (D + 3) / 4;
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (int64_t)W * H * G) return;
+    const int g = (int)(idx % G);
+    const int64_t pix = idx / G;
+    const int x = (int)(pix % W), y = (int)(pix / W);
+    const int d = 4 * g;
+    const uint8_t *cp = C + x * csx + y * csy + (int64_t)d * 4;
+    uint8_t *sp = S + x * ssx + y * ssy + (int64_t)d * 4;
+    const uint8_t *ep = planes + pix * D + d;
+    if constexpr (VEC) {
+        typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+        const u32x4 c = __builtin_nontemporal_load((const u32x4 *)cp);
+        uint32_t e0 = 0, e1 = 0, e2 = 0, e3 = 0;
+        for (int r = 0; r < n; r++) {
+            const uint32_t w = __builtin_nontemporal_load((const uint32_t *)(ep + (size_t)r * plane_bytes));
+            e0 += w & 0xFF;
+            e1 += (w >> 8) & 0xFF;
+            e2 += (w >> 16) & 0xFF;
+            e3 += w >> 24;
+        }
+        const u32x4 out = {wide_s(c.x, e0, n), wide_s(c.y, e1, n), wide_s(c.z, e2, n), wide_s(c.w, e3, n)};
+        *(u32x4 *)sp = out;
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (d + q >= D) break;
+            uint32_t esum = 0;
+            for (int r = 0; r < n; r++) esum += ep[(size_t)r * plane_bytes + q];
+            *(uint32_t *)(sp + 4 * q) = wide_s(*(const uint32_t *)(cp + 4 * q), esum, n);
+        }
+    }
+}
+
+template <int K>
+static void launch_wide_e_k(const WideEArgs &a, hipStream_t s)
+{
+    constexpr int PF = 8;
+    sgm_wide_e_kernel<K, PF><<<a.begin[a.ndirs], 64, 0, s>>>(a);
+}
+
+// The byte-plane form for uint32 volumes (P2 <= 255, planes below 2 GiB).
+static int run_wide_e(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params *p, const tdvh_dsi *S,
+                      hipStream_t s)
+{
+    const int W = C->width, H = C->height, D = C->depth;
+    WideEArgs a{};
+    a.C = (const uint8_t *)C->ptr;
+    a.csx = C->stride_x * 4;
+    a.csy = C->stride_y * 4;
+    a.plane_bytes = (size_t)W * H * D;
+    a.W = W;
+    a.H = H;
+    a.D = D;
+    a.P1 = (uint32_t)p->p1;
+    a.P2 = (uint32_t)p->p2;
+    // longest lines first: horizontal, vertical, then the diagonals
+    static const int order[8] = {0, 1, 2, 3, 4, 5, 6, 7};
+    const bool wide = W >= H;
+    int ord[8];
+    for (int i = 0; i < 8; i++) ord[i] = order[i];
+    if (!wide) { ord[0] = 2; ord[1] = 3; ord[2] = 0; ord[3] = 1; }
+    a.begin[0] = 0;
+    for (int i = 0; i < 8; i++) {
+        const int dir = ord[i];
+        if (!(p->dir_mask & (1 << dir))) continue;
+        a.dir[a.ndirs] = dir;
+        a.begin[a.ndirs + 1] = a.begin[a.ndirs] + dir_lines(dir, W, H);
+        a.ndirs++;
+    }
+    if (a.ndirs == 0) return TDVH_OK;
+    void *pb = nullptr;
+    TDVH_TRY(ctx->scratch("sgm_wide_planes", a.plane_bytes * a.ndirs, &pb));
+    a.planes = (uint8_t *)pb;
+    prof_begin(s);
+    switch ((D + 63) / 64) {
+    case 1: launch_wide_e_k<1>(a, s); break;
+    case 2: launch_wide_e_k<2>(a, s); break;
+    case 3: launch_wide_e_k<3>(a, s); break;
+    default: launch_wide_e_k<4>(a, s); break;
+    }
+    TDVH_TRY(check_launch("sgm_wide_lines_kernel", s));
+    const int64_t threads = (int64_t)W * H * ((D + 3) / 4);
+    const unsigned blocks = (unsigned)((threads + 255) / 256);
+    const bool vec = D % 4 == 0 && C->stride_x % 4 == 0 && C->stride_y % 4 == 0 &&
+                     S->stride_x % 4 == 0 && S->stride_y % 4 == 0 && ((uintptr_t)C->ptr & 15) == 0 &&
+                     ((uintptr_t)S->ptr & 15) == 0;
+    prof_begin(s);
+    if (vec)
+        sgm_wide_sum_kernel<true><<<blocks, 256, 0, s>>>(a.C, a.csx, a.csy, a.planes, a.plane_bytes,
+                                                         a.ndirs, (uint8_t *)S->ptr, S->stride_x * 4,
+                                                         S->stride_y * 4, W, H, D);
+    else
+        sgm_wide_sum_kernel<false><<<blocks, 256, 0, s>>>(a.C, a.csx, a.csy, a.planes, a.plane_bytes,
+                                                          a.ndirs, (uint8_t *)S->ptr, S->stride_x * 4,
+                                                          S->stride_y * 4, W, H, D);
+    return check_launch("sgm_wide_sum_kernel", s);
+}
+
 template <typename T>
 static int run_wide(const WideArgs<T> &a, int dir_mask, hipStream_t s)
 {
@@ -200,8 +453,11 @@ static int run_wide(const WideArgs<T> &a, int dir_mask, hipStream_t s)
     return TDVH_OK;
 }
 
-int run_sgm_wide(const tdvh_dsi *C, const tdvh_sgm_params *p, const tdvh_dsi *S, hipStream_t s)
+int run_sgm_wide(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params *p, const tdvh_dsi *S,
+                 hipStream_t s)
 {
+    if (C->dtype == TDVH_U32 && p->p2 <= 255 && (size_t)C->width * C->height * C->depth < ((size_t)1 << 31))
+        return run_wide_e(ctx, C, p, S, s);
     if (C->dtype == TDVH_U32) {
         WideArgs<uint32_t> a{(const uint32_t *)C->ptr, C->stride_x, C->stride_y,
                              (uint32_t *)S->ptr,       S->stride_x, S->stride_y,
