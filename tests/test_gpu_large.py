@@ -111,14 +111,50 @@ print("forced ok")
 
 
 @pytest.mark.parametrize("env", [{"TDVH_SGM_LINES": "flat"}, {"TDVH_SGM_SUM": "narrow"},
-                                 {"TDVH_SGM_LINES": "flat", "TDVH_SGM_SUM": "narrow"}],
-                         ids=["flat_lines", "narrow_sum", "both"])
+                                 {"TDVH_SGM_LINES": "flat", "TDVH_SGM_SUM": "narrow"},
+                                 {"TDVH_SGM_REL": "0"}, {"TDVH_SGM_REL": "1"}],
+                         ids=["flat_lines", "narrow_sum", "both", "bt_scans", "rel_scans"])
 def test_forced_fallback_kernels(env):
-    """The flat-address line kernel (taken for planes of 2 GiB and more) and
-    the one-row-per-pixel plane sum, forced on in a child process (the
-    library reads these switches once per process)."""
+    """The flat-address line kernel (taken for planes of 2 GiB and more),
+    the one-row-per-pixel plane sum, and the line scans with BT recomputed
+    (TDVH_SGM_REL=0) or over relative cost bytes (=1) wherever either
+    applies, forced in a child process (the library reads these switches
+    once per process)."""
     e = dict(os.environ)
     e.update(env)
     r = subprocess.run([sys.executable, "-c", _FORCED % ROOT], env=e, capture_output=True,
                        text=True, timeout=100)
     assert r.returncode == 0 and "forced ok" in r.stdout, r.stdout + r.stderr
+
+
+_REL_FULL = r"""
+import sys
+sys.path.insert(0, %r)
+import numpy as np, torch
+import oracle
+import thundervision_amd.tdvh as tv
+from thundervision_amd import synth
+dev = torch.device("cuda:0")
+for spec in (synth.WINDOW_1080P,
+             dict(width=1280, height=720, dmax=191, nrect=12, depth=192, seed=2008)):
+    L, R, _ = synth.workload_pair(spec)
+    D = spec["depth"]
+    print("case", spec["width"], spec["height"], D, flush=True)
+    want = oracle.stereo_bt_sgm(L, R, D)
+    Ld, Rd = torch.from_numpy(L).to(dev), torch.from_numpy(R).to(dev)
+    for fusion in (tv.FUSE_COST, tv.FUSE_STAGED):
+        got = tv.stereo_bt_sgm(Ld, Rd, D, fusion=fusion).cpu().numpy()
+        assert (got == want).all(), (spec, fusion, int((got != want).sum()))
+print("rel full ok")
+"""
+
+
+def test_relative_cost_scans_at_full_size():
+    """BT + SGM-8 + WTA at 1920x1080 D=256 and 1280x720 D=192 against the
+    oracle with the line scans forced onto relative cost bytes (these
+    shapes take the BT scans by default)."""
+    e = dict(os.environ)
+    e["TDVH_SGM_REL"] = "1"
+    r = subprocess.run([sys.executable, "-c", _REL_FULL % ROOT], env=e, capture_output=True,
+                       text=True, timeout=110)
+    assert r.returncode == 0 and "rel full ok" in r.stdout, r.stdout + r.stderr

@@ -354,6 +354,62 @@ def test_sgm_restarts_after_an_all_invalid_pixel():
     assert (Sf[:, k + 1:] == tail.astype(np.float32)).all() and np.isinf(Sf[0, k]).all()
 
 
+# ---- relative costs: the path-cost increments depend on C only through -------
+# ---- min(C - Cmin, 2 P2) per pixel -------------------------------------------------
+
+def _relative_costs(C, P2):
+    """The device's relative cost form of a uint16 volume: per pixel
+    min(C - Cmin, 2 P2) over its valid cells, 2 P2 on invalid cells, and
+    0xFFFF on every cell of a pixel without a valid cell."""
+    valid = C != oracle.INV16
+    c = C.astype(np.int64)
+    cmin = np.where(valid, c, 1 << 20).min(axis=2, keepdims=True)
+    rel = np.where(valid, np.minimum(c - cmin, 2 * P2), 2 * P2)
+    rel = np.where(valid.any(axis=2, keepdims=True), rel, oracle.INV16)
+    return rel.astype(np.uint16), valid
+
+
+@pytest.mark.parametrize("P1,P2", [(8, 32), (0, 17), (40, 5), (100, 63), (3, 0), (8, 1)])
+def test_sgm_relative_costs_give_the_same_increments(P1, P2):
+    # every direction's E = L - C on the valid cells is the same from the
+    # true costs and from the relative costs (edge cells d > x invalid, a
+    # few scattered invalid cells, pixels without a valid cell, BT-range
+    # and far larger costs)
+    H, W, D = 9, 23, 12
+    C = rng.integers(0, 511, (H, W, D)).astype(np.uint16)
+    C[rng.random((H, W, D)) < 0.1] = rng.integers(600, 3000)
+    for x in range(W):
+        C[:, x, x + 1:] = oracle.INV16
+    C[rng.random((H, W, D)) < 0.05] = oracle.INV16
+    C[4, 7, :] = oracle.INV16
+    C[0, 3, :] = oracle.INV16
+    C[:, 0, 0] = np.where(np.arange(H) % 3 == 0, oracle.INV16, C[:, 0, 0])
+    rel, valid = _relative_costs(C, P2)
+    for direction in range(8):
+        e_true = oracle.sgm_lr(C, P1, P2, direction).astype(np.int64) - C
+        e_rel = oracle.sgm_lr(rel, P1, P2, direction).astype(np.int64) - rel
+        assert (e_true[valid] == e_rel[valid]).all(), direction
+        assert (e_true[valid] >= 0).all() and (e_true[valid] <= P2).all()
+
+
+def test_sgm_relative_costs_fail_with_a_tighter_clip():
+    # the clip at 2 P2 is needed: clipping one below it changes some
+    # increments on random volumes (so the test above can fail)
+    H, W, D, P1, P2 = 12, 40, 16, 8, 32
+    C = rng.integers(0, 511, (H, W, D)).astype(np.uint16)
+    valid = np.ones_like(C, bool)
+    c = C.astype(np.int64)
+    cmin = c.min(axis=2, keepdims=True)
+    changed = False
+    for clip in (2 * P2 - 1, P2):
+        tight = np.minimum(c - cmin, clip).astype(np.uint16)
+        for direction in range(8):
+            e_true = oracle.sgm_lr(C, P1, P2, direction).astype(np.int64) - C
+            e_t = oracle.sgm_lr(tight, P1, P2, direction).astype(np.int64) - tight
+            changed |= bool((e_true[valid] != e_t[valid]).any())
+    assert changed
+
+
 # ---- SGM over uint32 and float volumes -------------------------------------------
 
 def _u16_to_u32(C):

@@ -6,8 +6,12 @@
 //   LINES a group of 4 (or 8) consecutive lines of one SGM direction
 //   SUM   a run of pixels of a pair: plane sum + lowest-index WTA
 // Pairs are grouped by m (words per lane, scan_m(D)), the register shape
-// of the SGM code, so each class's kernels are compiled for their own m and
-// keep their occupancy; the classes run as parallel branches of the graph.
+// of the SGM code, and by the scans' cost source (BT recomputed from the
+// term planes, or relative cost bytes, chosen per pair as the per-pair path
+// chooses; the byte form runs at fusion depth 1), so each class's kernels
+// are compiled for their own shape and keep their occupancy; the classes
+// run as parallel branches of the graph. A byte class runs one more phase
+// before its line scans: the relative cost bytes of its pairs.
 //
 // Fusion depth (how many of the three phases share one launch per class):
 //   1  one launch per phase (BT of all pairs; per class the line scans, then
@@ -55,6 +59,8 @@ constexpr uint32_t SPIN_LIMIT = 1u << 24;
 // control words: a queue head per class, the error flag, then per pair the
 // BT and LINES completion counters
 constexpr int CTL_ERR = 9, CTL_PAIRS = 16;
+// classes: m + NM * (relative cost bytes)
+constexpr int NM = 9, NCLS = 2 * NM;
 
 // ---- per-item work (shared by the phase kernels and the persistent one) ----
 
@@ -81,6 +87,19 @@ __device__ __forceinline__ void run_sum_units(const BPair &P, int u0, int n)
     }
 }
 
+// Relative cost bytes of wave units [u0, u0 + n) (the units of the sums).
+template <int M>
+__device__ __forceinline__ void run_rel_units(const BPair &P, int u0, int n)
+{
+    constexpr int G = sum_group<M>();
+    const int lane = threadIdx.x & 63;
+    const long npix = (long)P.a.W * P.a.H;
+    for (int u = u0; u < u0 + n; u++) {
+        const long p = (long)u * (64 / G) + lane / G;
+        if (p < npix) sgm_rel_pixel<M, true>(P.a, p, lane % G);
+    }
+}
+
 // ---- fusion depth 1: one launch per phase ------------------------------------
 
 __global__ __launch_bounds__(256) void batch_bt_kernel(const BPair *__restrict__ pairs,
@@ -103,9 +122,9 @@ __global__ __launch_bounds__(256) void batch_lines_kernel(const BPair *__restric
     sgm_line_group<M, true, true, PF>(pairs[it.pair].a, it.a, it.b, it.a, prio);
 }
 
-// Buffer-addressed line groups of one depth class (LPL lanes per line);
-// waves_per_group waves per workgroup, two waves per SIMD.
-template <int M, bool PAD, int PF, int LPL>
+// Buffer-addressed line groups of one class (LPL lanes per line, cost
+// source SRC); waves_per_group waves per workgroup, two waves per SIMD.
+template <int M, int SRC, bool PAD, int PF, int LPL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void
 batch_lines_buf_kernel(const BPair *__restrict__ pairs, const BItem *__restrict__ items, int n,
                        int prio)
@@ -114,13 +133,14 @@ batch_lines_buf_kernel(const BPair *__restrict__ pairs, const BItem *__restrict_
                                                  (threadIdx.x >> 6));
     if (w >= n) return;
     const BItem it = items[w];
-    sgm_line_group_buf<M, true, PAD, PF, LPL>(pairs[it.pair].a, it.a, it.b, it.a, prio);
+    sgm_line_group_buf<M, SRC, PAD, PF, LPL>(pairs[it.pair].a, it.a, it.b, it.a, prio);
 }
 
-// Plane sum + WTA: items {pair, SUM, first wave, waves}, one per pair of the
-// class in increasing first wave; a wave finds its pair by binary search
+// Plane sum + WTA (REL false) or relative cost bytes (REL true) over the
+// class's SUM items {pair, SUM, first wave, waves}, one per pair in
+// increasing first wave; a wave finds its pair by binary search
 // (wave-uniform, scalar loads).
-template <int M>
+template <int M, bool REL>
 __global__ __launch_bounds__(256) void batch_sum_wide_kernel(const BPair *__restrict__ pairs,
                                                              const BItem *__restrict__ items,
                                                              int n, int waves)
@@ -134,7 +154,8 @@ __global__ __launch_bounds__(256) void batch_sum_wide_kernel(const BPair *__rest
         else hi = mid - 1;
     }
     const BItem it = items[lo];
-    run_sum_units<M>(pairs[it.pair], w - it.a, 1);
+    if constexpr (REL) run_rel_units<M>(pairs[it.pair], w - it.a, 1);
+    else run_sum_units<M>(pairs[it.pair], w - it.a, 1);
 }
 
 // ---- fusion depth 2 and 3: one persistent launch per class ----------------------
@@ -195,7 +216,7 @@ batch_queue_kernel(const BPair *__restrict__ pairs, const BItem *__restrict__ it
         if (it.kind == IT_BT) {
             run_bt_item(P, it);
         } else if (it.kind == IT_LINES) {
-            sgm_line_group_buf<M, true, PAD, PF, LPL>(P.a, it.a, it.b, it.a, prio);
+            sgm_line_group_buf<M, SRC_TERMS, PAD, PF, LPL>(P.a, it.a, it.b, it.a, prio);
             __builtin_amdgcn_s_setprio(0);
         } else {
             run_sum_units<M>(P, it.a, it.b);
@@ -223,6 +244,7 @@ struct tdvh_batch {
     tdvh_ctx *ctx = nullptr;
     int n = 0;
     std::vector<tdvh_pair> host_pairs;
+    std::vector<BPair> host_bp; // the device descriptors, host copy
     tdvh_sgm_params params{};
     void *mem = nullptr; // pair buffers, descriptors, control words
     size_t mem_bytes = 0;
@@ -239,7 +261,7 @@ struct tdvh_batch {
         int line_begin = 0, line_count = 0; // depth 1
         int sum_begin = 0, sum_count = 0, sum_waves = 0;
         int queue_begin = 0, queue_count = 0; // depth 2, 3
-    } cls[9];
+    } cls[NCLS];
     int bt_begin = 0, bt_count = 0; // depth 1, 2: BT items of every pair
     int64_t cells = 0;
     // knobs
@@ -249,8 +271,8 @@ struct tdvh_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t graph_stream = nullptr;
-    hipStream_t branch[9] = {}, sum_branch[9] = {};
-    hipEvent_t fork = nullptr, join[9] = {}, sum_join[9] = {}, lines_done[9] = {};
+    hipStream_t branch[NCLS] = {};
+    hipEvent_t fork = nullptr, join[NCLS] = {};
 };
 
 static void drop_graph(tdvh_batch *b)
@@ -291,12 +313,19 @@ static int pair_sum_units(const tdvh_pair &p, int mc)
     return (int)((npix * G + 63) / 64);
 }
 
-// Lines per wave usable for class mc: 8 needs 8-lane lines (2m words per
+// Class of pair i under the current fusion depth.
+static int pair_class(const tdvh_batch *b, int i)
+{
+    const BPair &q = b->host_bp[i];
+    return q.m + (b->depth == 1 && q.a.CR && buf_lines_fit(q.a) ? NM : 0);
+}
+
+// Lines per wave usable for class ci: 8 needs 8-lane lines (2m words per
 // lane, at most 4, which keeps the prefetch ring within two waves per SIMD)
 // and the buffer kernel.
-static int class_lpw(const tdvh_batch *b, int mc)
+static int class_lpw(const tdvh_batch *b, int ci)
 {
-    return (b->lines_per_wave == 8 && mc <= 2 && b->cls[mc].buf) ? 8 : 4;
+    return (b->lines_per_wave == 8 && ci % NM <= 2 && b->cls[ci].buf) ? 8 : 4;
 }
 
 // Builds the item lists for the current knobs and uploads them (the batch
@@ -317,20 +346,37 @@ static int plan_items(tdvh_batch *b)
     if (b->depth < 3)
         for (int i = 0; i < n; i++) bt_items(i);
     b->bt_count = (int)items.size() - b->bt_begin;
-    for (int mc = 1; mc <= 8; mc++) {
-        tdvh_batch::Class &c = b->cls[mc];
+    // class flags under the current depth
+    for (tdvh_batch::Class &c : b->cls) c = tdvh_batch::Class{};
+    for (int i = 0; i < n; i++) {
+        const BPair &q = b->host_bp[i];
+        tdvh_batch::Class &c = b->cls[pair_class(b, i)];
+        c.used = true;
+        if (q.a.D < 32 * q.m) c.pad = true;
+        if (!buf_lines_fit(q.a)) c.buf = false;
+    }
+    for (int ci = 1; ci < NCLS; ci++) {
+        tdvh_batch::Class &c = b->cls[ci];
         if (!c.used) continue;
+        const int mc = ci % NM;
         std::vector<int> members;
         for (int i = 0; i < n; i++)
-            if (scan_m(b->host_pairs[i].depth) == mc) members.push_back(i);
-        const int lpw = class_lpw(b, mc);
-        const bool persistent = b->depth >= 2 && c.buf;
+            if (pair_class(b, i) == ci) members.push_back(i);
+        const int lpw = class_lpw(b, ci);
+        const bool persistent = b->depth >= 2 && ci < NM && c.buf;
         if (!persistent) {
+            // each pair's line groups in the per-pair path's chiplet-banded,
+            // longest-first order for this launch's waves per workgroup
             c.line_begin = (int)items.size();
+            std::vector<uint32_t> tab;
             for (int i : members) {
-                const size_t before = items.size();
-                pair_line_items(b->host_pairs[i], i, lpw, items);
-                need[n + i] = (int)(items.size() - before);
+                const tdvh_pair &p = b->host_pairs[i];
+                const size_t before = tab.size();
+                line_order_append(p.width, p.height, 0xFF, lpw, b->waves_per_group, tab);
+                for (size_t k = before; k < tab.size(); k++)
+                    items.push_back({i, IT_LINES, (int)((tab[k] >> 24) & 0xF),
+                                     (int)(tab[k] & 0xFFFFFF)});
+                need[n + i] = (int)(tab.size() - before);
             }
             c.line_count = (int)items.size() - c.line_begin;
             c.sum_begin = (int)items.size();
@@ -405,7 +451,7 @@ extern "C" int tdvh_batch_create(tdvh_ctx *ctx, const tdvh_pair *pairs, int n,
     TDVH_REQUIRE(params->p2 >= 0 && params->p2 <= 255,
                  "batches need 0 <= P2 <= 255 (byte path-cost planes)");
     std::vector<BPair> bp(n);
-    std::vector<size_t> off_terms(n), off_planes(n);
+    std::vector<size_t> off_terms(n), off_planes(n), off_rel(n);
     size_t total = 0;
     auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
     int64_t cells = 0;
@@ -421,6 +467,11 @@ extern "C" int tdvh_batch_create(tdvh_ctx *ctx, const tdvh_pair *pairs, int n,
         total = align(total + (W * H + 3 * (size_t)Wr * H) * 4);
         off_planes[i] = total;
         total = align(total + 8 * W * H * 32 * m);
+        off_rel[i] = 0;
+        if (sgm_rel_auto(p.width, p.height, m, params->p2)) {
+            off_rel[i] = total;
+            total = align(total + W * H * 32 * m);
+        }
         cells += (int64_t)W * H * p.depth;
     }
     const size_t trash_off = total;
@@ -458,6 +509,7 @@ extern "C" int tdvh_batch_create(tdvh_ctx *ctx, const tdvh_pair *pairs, int n,
         q.a.planes = base + off_planes[i];
         q.a.plane_bytes = (size_t)p.width * p.height * 32 * m;
         q.a.trash = base + trash_off;
+        q.a.CR = off_rel[i] ? base + off_rel[i] : nullptr;
         q.L = p.left;
         q.R = p.right;
         q.pitch = p.pitch;
@@ -472,12 +524,7 @@ extern "C" int tdvh_batch_create(tdvh_ctx *ctx, const tdvh_pair *pairs, int n,
     b->d_pairs = (BPair *)(base + pairs_off);
     b->d_ctl = (uint32_t *)(base + ctl_off);
     b->nctl = nctl;
-    for (int i = 0; i < n; i++) {
-        tdvh_batch::Class &c = b->cls[bp[i].m];
-        c.used = true;
-        if (bp[i].a.D < 32 * bp[i].m) c.pad = true;
-        if (!buf_lines_fit(bp[i].a)) c.buf = false;
-    }
+    b->host_bp = bp;
     b->cells = cells;
     int rc = TDVH_OK;
     if (hipMemcpy(b->d_pairs, bp.data(), sizeof(BPair) * n, hipMemcpyHostToDevice) != hipSuccess) {
@@ -503,12 +550,9 @@ extern "C" int tdvh_batch_destroy(tdvh_batch *b)
     TDVH_CHECK_HIP(hipDeviceSynchronize());
     drop_graph(b);
     if (b->graph_stream) (void)hipStreamDestroy(b->graph_stream);
-    for (int mc = 0; mc <= 8; mc++) {
-        if (b->branch[mc]) (void)hipStreamDestroy(b->branch[mc]);
-        if (b->sum_branch[mc]) (void)hipStreamDestroy(b->sum_branch[mc]);
-        if (b->join[mc]) (void)hipEventDestroy(b->join[mc]);
-        if (b->sum_join[mc]) (void)hipEventDestroy(b->sum_join[mc]);
-        if (b->lines_done[mc]) (void)hipEventDestroy(b->lines_done[mc]);
+    for (int ci = 0; ci < NCLS; ci++) {
+        if (b->branch[ci]) (void)hipStreamDestroy(b->branch[ci]);
+        if (b->join[ci]) (void)hipEventDestroy(b->join[ci]);
     }
     if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->d_need) (void)hipFree(b->d_need);
@@ -536,7 +580,8 @@ extern "C" int tdvh_batch_config(tdvh_batch *b, int lines_per_wave, int waves_pe
     const int pf = prefetch ? prefetch : b->prefetch;
     const int dep = fusion_depth ? fusion_depth : b->depth;
     const int lag = sum_lag ? sum_lag : b->sum_lag;
-    const bool replan = lpw != b->lines_per_wave || dep != b->depth || lag != b->sum_lag;
+    const bool replan = lpw != b->lines_per_wave || dep != b->depth || lag != b->sum_lag ||
+                        wpg != b->waves_per_group;
     if (replan || wpg != b->waves_per_group || pf != b->prefetch) drop_graph(b);
     b->lines_per_wave = lpw;
     b->waves_per_group = wpg;
@@ -551,39 +596,56 @@ extern "C" int tdvh_batch_config(tdvh_batch *b, int lines_per_wave, int waves_pe
     return TDVH_OK;
 }
 
-template <int M, bool PAD, int PF>
+template <int M, int SRC, bool PAD, int PF>
 static void launch_lines_buf(tdvh_batch *b, int lpl, const BItem *li, int nl, int blocks,
                              int threads, hipStream_t s)
 {
     if constexpr (M <= 2) {
         if (lpl == 8) {
-            batch_lines_buf_kernel<M, PAD, PF, 8><<<blocks, threads, 0, s>>>(b->d_pairs, li, nl,
-                                                                             b->prio);
+            batch_lines_buf_kernel<M, SRC, PAD, PF, 8><<<blocks, threads, 0, s>>>(b->d_pairs, li,
+                                                                                  nl, b->prio);
             return;
         }
     }
-    batch_lines_buf_kernel<M, PAD, PF, 16><<<blocks, threads, 0, s>>>(b->d_pairs, li, nl, b->prio);
+    batch_lines_buf_kernel<M, SRC, PAD, PF, 16><<<blocks, threads, 0, s>>>(b->d_pairs, li, nl,
+                                                                           b->prio);
 }
 
 template <int M>
-static void launch_lines_class(tdvh_batch *b, const tdvh_batch::Class &c, int mc, hipStream_t s)
+static void launch_lines_class(tdvh_batch *b, const tdvh_batch::Class &c, int ci, hipStream_t s)
 {
     const int threads = 64 * b->waves_per_group, wpb = b->waves_per_group;
     const int nl = c.line_count;
     if (!nl) return;
     const BItem *li = b->d_items + c.line_begin;
     const int blocks = (nl + wpb - 1) / wpb;
-    const int lpl = 64 / class_lpw(b, mc);
+    const int lpl = 64 / class_lpw(b, ci);
+    if constexpr (rel_ok<M>()) {
+        if (ci >= NM) {
+            // relative cost bytes of the class's pairs, then the byte scans
+            // (16 steps in flight, as the per-pair path)
+            prof_begin(s);
+            batch_sum_wide_kernel<M, true><<<(c.sum_waves + 3) / 4, 256, 0, s>>>(
+                b->d_pairs, b->d_items + c.sum_begin, c.sum_count, c.sum_waves);
+            (void)check_launch("batch_rel_kernel", s);
+            prof_begin(s);
+            launch_lines_buf<M, SRC_REL, false, 16>(b, lpl, li, nl, blocks, threads, s);
+            (void)check_launch("batch_lines_kernel", s);
+            return;
+        }
+    }
     prof_begin(s);
     if (c.buf) {
         // "prefetch 8": the deepest ring that keeps two waves per SIMD
         constexpr int PF8 = buf_lines_pf<M, true>();
         if (c.pad) {
-            if (b->prefetch == 8) launch_lines_buf<M, true, PF8>(b, lpl, li, nl, blocks, threads, s);
-            else launch_lines_buf<M, true, 4>(b, lpl, li, nl, blocks, threads, s);
+            if (b->prefetch == 8)
+                launch_lines_buf<M, SRC_TERMS, true, PF8>(b, lpl, li, nl, blocks, threads, s);
+            else launch_lines_buf<M, SRC_TERMS, true, 4>(b, lpl, li, nl, blocks, threads, s);
         } else {
-            if (b->prefetch == 8) launch_lines_buf<M, false, PF8>(b, lpl, li, nl, blocks, threads, s);
-            else launch_lines_buf<M, false, 4>(b, lpl, li, nl, blocks, threads, s);
+            if (b->prefetch == 8)
+                launch_lines_buf<M, SRC_TERMS, false, PF8>(b, lpl, li, nl, blocks, threads, s);
+            else launch_lines_buf<M, SRC_TERMS, false, 4>(b, lpl, li, nl, blocks, threads, s);
         }
     } else {
         if (b->prefetch == 8)
@@ -600,8 +662,8 @@ static void launch_sum_class(tdvh_batch *b, const tdvh_batch::Class &c, hipStrea
     if (!c.sum_count) return;
     const BItem *si = b->d_items + c.sum_begin;
     prof_begin(s);
-    batch_sum_wide_kernel<M><<<(c.sum_waves + 3) / 4, 256, 0, s>>>(b->d_pairs, si, c.sum_count,
-                                                                    c.sum_waves);
+    batch_sum_wide_kernel<M, false><<<(c.sum_waves + 3) / 4, 256, 0, s>>>(b->d_pairs, si,
+                                                                           c.sum_count, c.sum_waves);
     (void)check_launch("batch_sum_kernel", s);
 }
 
@@ -625,51 +687,55 @@ static void launch_queue_pf(tdvh_batch *b, int lpl, const tdvh_batch::Class &c, 
 // One persistent launch for the class: as many waves as stay resident at two
 // per SIMD (fewer for short queues).
 template <int M>
-static void launch_queue_class(tdvh_batch *b, const tdvh_batch::Class &c, int mc, hipStream_t s)
+static void launch_queue_class(tdvh_batch *b, const tdvh_batch::Class &c, int ci, hipStream_t s)
 {
     if (!c.queue_count) return;
     const int wpb = b->waves_per_group, threads = 64 * wpb;
     const int resident = 2 * 4 * std::max(b->ctx->num_cus, 1);
     const int waves = std::min(c.queue_count, resident);
     const int grid = (waves + wpb - 1) / wpb;
-    const int lpl = 64 / class_lpw(b, mc);
+    const int lpl = 64 / class_lpw(b, ci);
     constexpr int PF8 = buf_lines_pf<M, true>();
     prof_begin(s);
     if (c.pad) {
-        if (b->prefetch == 8) launch_queue_pf<M, true, PF8>(b, lpl, c, mc, grid, threads, s);
-        else launch_queue_pf<M, true, 4>(b, lpl, c, mc, grid, threads, s);
+        if (b->prefetch == 8) launch_queue_pf<M, true, PF8>(b, lpl, c, ci, grid, threads, s);
+        else launch_queue_pf<M, true, 4>(b, lpl, c, ci, grid, threads, s);
     } else {
-        if (b->prefetch == 8) launch_queue_pf<M, false, PF8>(b, lpl, c, mc, grid, threads, s);
-        else launch_queue_pf<M, false, 4>(b, lpl, c, mc, grid, threads, s);
+        if (b->prefetch == 8) launch_queue_pf<M, false, PF8>(b, lpl, c, ci, grid, threads, s);
+        else launch_queue_pf<M, false, 4>(b, lpl, c, ci, grid, threads, s);
     }
     (void)check_launch("batch_queue_kernel", s);
 }
 
-// Phase `what` (0 lines, 1 sums, 2 persistent queue) of class mc on `s`.
-static int launch_class(tdvh_batch *b, int mc, int what, hipStream_t s)
+// Phase `what` (0 lines, 1 sums, 2 persistent queue) of class ci on `s`.
+static int launch_class(tdvh_batch *b, int ci, int what, hipStream_t s)
 {
-    const tdvh_batch::Class &c = b->cls[mc];
-    switch (mc) {
+    const tdvh_batch::Class &c = b->cls[ci];
+    switch (ci % NM) {
 #define CASE(m)                                                                                    \
     case m:                                                                                        \
-        if (what == 0) launch_lines_class<m>(b, c, mc, s);                                         \
+        if (what == 0) launch_lines_class<m>(b, c, ci, s);                                         \
         else if (what == 1) launch_sum_class<m>(b, c, s);                                          \
-        else launch_queue_class<m>(b, c, mc, s);                                                   \
+        else launch_queue_class<m>(b, c, ci, s);                                                   \
         break;
         CASE(1) CASE(2) CASE(4) CASE(6) CASE(8)
 #undef CASE
-    default: set_error("no batch kernels for %d words per lane", mc); return TDVH_ERR_UNSUPPORTED;
+    default:
+        set_error("no batch kernels for %d words per lane", ci % NM);
+        return TDVH_ERR_UNSUPPORTED;
     }
     return check_launch("batch_class", s);
 }
 
-static bool class_persistent(const tdvh_batch *b, int mc)
+// Persistent queues: BT-scan classes at fusion depth 2 and 3 (the classes
+// of relative cost bytes exist at depth 1 only).
+static bool class_persistent(const tdvh_batch *b, int ci)
 {
-    return b->depth >= 2 && b->cls[mc].buf;
+    return b->depth >= 2 && ci < NM && b->cls[ci].buf;
 }
 
-// Enqueues the batch on `s`. With fork (graph capture) each depth class runs
-// on branch streams forked from and joined back into `s`.
+// Enqueues the batch on `s`. With fork (graph capture) each class runs on
+// branch streams forked from and joined back into `s`.
 static int enqueue_batch(tdvh_batch *b, hipStream_t s, bool fork)
 {
     if (b->depth >= 2) {
@@ -684,35 +750,35 @@ static int enqueue_batch(tdvh_batch *b, hipStream_t s, bool fork)
         TDVH_TRY(check_launch("batch_bt_kernel", s));
     }
     if (!fork) {
-        for (int mc = 1; mc <= 8; mc++) {
-            if (!b->cls[mc].used) continue;
-            if (class_persistent(b, mc)) {
-                TDVH_TRY(launch_class(b, mc, 2, s));
+        for (int ci = 1; ci < NCLS; ci++) {
+            if (!b->cls[ci].used) continue;
+            if (class_persistent(b, ci)) {
+                TDVH_TRY(launch_class(b, ci, 2, s));
             } else {
-                TDVH_TRY(launch_class(b, mc, 0, s));
-                TDVH_TRY(launch_class(b, mc, 1, s));
+                TDVH_TRY(launch_class(b, ci, 0, s));
+                TDVH_TRY(launch_class(b, ci, 1, s));
             }
         }
         return TDVH_OK;
     }
     if (!b->fork) TDVH_CHECK_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
     TDVH_CHECK_HIP(hipEventRecord(b->fork, s));
-    for (int mc = 1; mc <= 8; mc++) {
-        if (!b->cls[mc].used) continue;
-        if (!b->branch[mc])
-            TDVH_CHECK_HIP(hipStreamCreateWithFlags(&b->branch[mc], hipStreamNonBlocking));
-        if (!b->join[mc])
-            TDVH_CHECK_HIP(hipEventCreateWithFlags(&b->join[mc], hipEventDisableTiming));
-        hipStream_t ls = b->branch[mc];
+    for (int ci = 1; ci < NCLS; ci++) {
+        if (!b->cls[ci].used) continue;
+        if (!b->branch[ci])
+            TDVH_CHECK_HIP(hipStreamCreateWithFlags(&b->branch[ci], hipStreamNonBlocking));
+        if (!b->join[ci])
+            TDVH_CHECK_HIP(hipEventCreateWithFlags(&b->join[ci], hipEventDisableTiming));
+        hipStream_t ls = b->branch[ci];
         TDVH_CHECK_HIP(hipStreamWaitEvent(ls, b->fork, 0));
-        if (class_persistent(b, mc)) {
-            TDVH_TRY(launch_class(b, mc, 2, ls));
+        if (class_persistent(b, ci)) {
+            TDVH_TRY(launch_class(b, ci, 2, ls));
         } else {
-            TDVH_TRY(launch_class(b, mc, 0, ls));
-            TDVH_TRY(launch_class(b, mc, 1, ls));
+            TDVH_TRY(launch_class(b, ci, 0, ls));
+            TDVH_TRY(launch_class(b, ci, 1, ls));
         }
-        TDVH_CHECK_HIP(hipEventRecord(b->join[mc], ls));
-        TDVH_CHECK_HIP(hipStreamWaitEvent(s, b->join[mc], 0));
+        TDVH_CHECK_HIP(hipEventRecord(b->join[ci], ls));
+        TDVH_CHECK_HIP(hipStreamWaitEvent(s, b->join[ci], 0));
     }
     return TDVH_OK;
 }

@@ -28,6 +28,17 @@ int run_sgm_wide(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params *p, con
 
 int popcount8(int mask);
 
+// Whether the fused BT + SGM scans of a W x H pair at m words per lane and
+// penalty P2 read relative cost bytes (sgm_rel_pixel) rather than
+// recomputing BT (the choice of the per-pair path, sgm.hip).
+bool sgm_rel_auto(int W, int H, int m, int p2);
+
+// Appends the line-group table of one W x H pair (entries plane slot << 28 |
+// direction << 24 | first line; lpw lines per group) to `out`, ordered
+// longest first within chiplet bands for wpg waves per workgroup, entry k of
+// the launch running on workgroup k / wpg (sgm.hip).
+void line_order_append(int W, int H, int dir_mask, int lpw, int wpg, std::vector<uint32_t> &out);
+
 // Frees the reference-geometry tables cached for a context.
 void release_ref_tables(const tdvh_ctx *ctx);
 

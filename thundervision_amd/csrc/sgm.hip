@@ -27,6 +27,12 @@
 
 namespace tdvh {
 
+static int env_int(const char *name, int dflt)
+{
+    const char *e = getenv(name);
+    return e ? atoi(e) : dflt;
+}
+
 int popcount8(int mask)
 {
     int n = 0;
@@ -72,13 +78,54 @@ static void launch_lines_buf(const SgmArgs &a, const uint32_t *order, int groups
     else sgm_lines_buf_kernel<M, FUSED, false, 16><<<groups, 64, 0, s>>>(a, order, prio);
 }
 
+// Whether the scans read relative cost bytes (sgm_rel_pixel): automatic,
+// or forced by TDVH_SGM_REL=1 (on where applicable) / 0 (off).
+//
+// Measured (BT + SGM-8 + WTA per pair, fused): the byte scans move 1 B per
+// cell and direction from memory instead of recomputing BT on the vector
+// ALUs. That pays where the BT step's fixed costs dominate (1 or 2 words
+// per lane: D <= 64, 14-20 % faster at 1280x720 and 1920x1080) and where
+// the bytes stay in the device's caches (640x480, D = 128..256: 10-15 %
+// faster). At 118 MB of cost bytes (1280x720, D = 128) the result varies
+// from device to device (-2 % to +7 %), and beyond (1280x720 D = 256,
+// 1920x1080 D >= 128) the byte scans are 10-20 % slower.
+static bool rel_wanted(int W, int H, int m)
+{
+    static const int mode = [] {
+        const char *e = getenv("TDVH_SGM_REL");
+        return e ? atoi(e) : -1;
+    }();
+    if (mode >= 0) return mode != 0;
+    return m <= 2 || (size_t)W * H * 32 * m <= ((size_t)96 << 20);
+}
+
+bool sgm_rel_auto(int W, int H, int m, int p2)
+{
+    return p2 <= 63 && (m == 1 || m % 2 == 0) && rel_wanted(W, H, m);
+}
+
+template <int M>
+static void launch_lines_rel(const SgmArgs &a, const uint32_t *order, int groups, int prio,
+                             bool lanes8, hipStream_t s)
+{
+    if constexpr (rel_ok<M>()) {
+        if constexpr (M <= 4) {
+            if (lanes8) {
+                sgm_lines_rel_kernel<M, 8><<<groups, 64, 0, s>>>(a, order, prio);
+                return;
+            }
+        }
+        sgm_lines_rel_kernel<M, 16><<<groups, 64, 0, s>>>(a, order, prio);
+    }
+}
+
 // 8-lane lines (8 per wave, 4 words per lane) for the fused scans at 2 words
 // per lane on large images: measured 12 % faster at 1920x1080 (D = 64), but
 // slower where the launch is short and its longest lines set the time
 // (640x480: +3 %), and slower at 1 word per lane (1280x720, D = 32: +6 %).
 static bool lines_8lane(const SgmArgs &a)
 {
-    return a.C == nullptr && a.m == 2 && (long)a.W * a.H >= 1536L * 1024;
+    return (a.C == nullptr || a.CR != nullptr) && a.m == 2 && (long)a.W * a.H >= 1536L * 1024;
 }
 
 template <int M, bool FUSED, bool EOUT>
@@ -103,11 +150,6 @@ static void launch_lines_m(const SgmArgs &a, const LineSet &ls, bool eout, hipSt
     else launch_lines_k<M, false, false>(a, ls, s);
 }
 
-static int env_int(const char *name, int dflt)
-{
-    const char *e = getenv(name);
-    return e ? atoi(e) : dflt;
-}
 
 static LineSet make_lineset(int W, int H, int dir_mask)
 {
@@ -140,22 +182,22 @@ static int line_key(int dir, int sx, int sy)
 // launch runs on chiplet b % NXCD.
 constexpr int NXCD = 8;
 
-// Device table of the line groups (lpw consecutive lines of one direction:
-// the lines of one wave, 4, or 8 with 8-lane lines) of every enabled
-// direction. Entry: plane slot << 28 | direction << 24 | first line;
-// workgroup b runs entry b. Two goals:
-//  * longest first: with one wave per workgroup the dispatcher then runs a
+// Line-group table of one pair (lpw consecutive lines of one direction: the
+// lines of one wave, 4, or 8 with 8-lane lines) over the enabled
+// directions, appended to `out`. Entry: plane slot << 28 | direction << 24
+// | first line. Entry k of the launch (k = out's index, counting the
+// entries already there) runs on workgroup k / wpg, which the hardware
+// places on chiplet (k / wpg) mod NXCD. Two goals:
+//  * longest first: the dispatcher then runs a
 //    longest-processing-time-first list schedule;
 //  * chiplet locality: each direction's groups, in spatial order, are cut
 //    into NXCD bands of equal work and band c goes to chiplet c, so the
 //    waves sharing rows of the term planes share one L2 (round-robin
 //    placement would make every chiplet fetch every row).
-// Entry b is the next entry of chiplet b % NXCD's longest-first queue.
-// Built and uploaded once per (W, H, dir_mask) and context.
-static int line_order(tdvh_ctx *ctx, int W, int H, int dir_mask, int lpw, const uint32_t **out,
-                      int *groups, hipStream_t s)
+// Entry k is the next entry of its chiplet's longest-first queue (another
+// chiplet's longest remaining one once that queue is empty).
+void line_order_append(int W, int H, int dir_mask, int lpw, int wpg, std::vector<uint32_t> &out)
 {
-    TDVH_REQUIRE(W + H < (1 << 24), "image %dx%d too large for the line table", W, H);
     struct Group {
         int len, key;
         uint32_t entry;
@@ -196,11 +238,9 @@ static int line_order(tdvh_ctx *ctx, int W, int H, int dir_mask, int lpw, const 
                          [](const std::pair<int, uint32_t> &x, const std::pair<int, uint32_t> &y) {
                              return x.first > y.first;
                          });
-    std::vector<uint32_t> h;
-    h.reserve(total);
     size_t head[NXCD] = {};
     for (size_t b = 0; b < total; b++) {
-        int c = (int)(b % NXCD);
+        int c = (int)((out.size() / std::max(wpg, 1)) % NXCD);
         if (head[c] == q[c].size()) {
             // this chiplet's queue is drained: take the longest remaining head
             int best = -1;
@@ -210,30 +250,42 @@ static int line_order(tdvh_ctx *ctx, int W, int H, int dir_mask, int lpw, const 
                     best = o;
             c = best;
         }
-        h.push_back(q[c][head[c]++].second);
+        out.push_back(q[c][head[c]++].second);
     }
+}
+
+// The table of line_order_append for one pair and one wave per workgroup,
+// built and uploaded once per (W, H, dir_mask, lpw) and context.
+static int line_order(tdvh_ctx *ctx, int W, int H, int dir_mask, int lpw, const uint32_t **out,
+                      int *groups, hipStream_t s)
+{
+    TDVH_REQUIRE(W + H < (1 << 24), "image %dx%d too large for the line table", W, H);
     // one table per shape, mask and lines per wave: a context serving several
     // shapes keeps them all, each in its own buffer that no other shape
     // reuses, so the upload is ordered on the launch stream (no device-wide
-    // synchronisation) and steady-state calls never upload
+    // synchronisation) and steady-state calls never rebuild or upload
     char name[64];
     snprintf(name, sizeof name, "sgm_line_order:%dx%d:%02x:%d", W, H, dir_mask, lpw);
+    const uint64_t tag = ((uint64_t)W << 40) | ((uint64_t)H << 16) | ((uint64_t)lpw << 8) |
+                         (uint64_t)dir_mask;
+    std::vector<uint32_t> &host = ctx->host_tables[name];
+    tdvh_ctx::Slot &sl0 = ctx->slot(name);
+    if (sl0.tag == tag && sl0.ptr) {
+        *groups = (int)host.size();
+        *out = (const uint32_t *)sl0.ptr;
+        return TDVH_OK;
+    }
+    std::vector<uint32_t> h;
+    line_order_append(W, H, dir_mask, lpw, 1, h);
     void *buf = nullptr;
     TDVH_TRY(ctx->scratch(name, h.size() * 4, &buf));
     tdvh_ctx::Slot &sl = ctx->slot(name);
-    const uint64_t tag = ((uint64_t)W << 40) | ((uint64_t)H << 16) | ((uint64_t)lpw << 8) |
-                         (uint64_t)dir_mask;
-    if (sl.tag != tag) {
-        // the host copy lives in the context for as long as the buffer, so
-        // the asynchronous copy's source stays valid
-        std::vector<uint32_t> &host = ctx->host_tables[name];
-        host = std::move(h);
-        TDVH_CHECK_HIP(hipMemcpyAsync(buf, host.data(), host.size() * 4, hipMemcpyHostToDevice, s));
-        sl.tag = tag;
-        *groups = (int)host.size();
-    } else {
-        *groups = (int)h.size();
-    }
+    // the host copy lives in the context for as long as the buffer, so the
+    // asynchronous copy's source stays valid
+    host = std::move(h);
+    TDVH_CHECK_HIP(hipMemcpyAsync(buf, host.data(), host.size() * 4, hipMemcpyHostToDevice, s));
+    sl.tag = tag;
+    *groups = (int)host.size();
     *out = (const uint32_t *)buf;
     return TDVH_OK;
 }
@@ -246,6 +298,7 @@ int launch_sgm_lines(tdvh_ctx *ctx, const SgmArgs &a, int dir_mask, bool eout, h
         const uint32_t *order = nullptr;
         int groups = 0;
         const bool fused = a.C == nullptr;
+        const bool rel = a.CR != nullptr;
         const bool lanes8 = lines_8lane(a);
         const int lpw = lanes8 ? 8 : 4; // lines per wave
         TDVH_TRY(line_order(ctx, a.W, a.H, dir_mask, lpw, &order, &groups, s));
@@ -253,7 +306,8 @@ int launch_sgm_lines(tdvh_ctx *ctx, const SgmArgs &a, int dir_mask, bool eout, h
         switch (a.m) {
 #define CASE(m)                                                                                    \
     case m:                                                                                        \
-        if (fused) launch_lines_buf<m, true>(a, order, groups, ls.prio, lanes8, s);                \
+        if (rel) launch_lines_rel<m>(a, order, groups, ls.prio, lanes8, s);                        \
+        else if (fused) launch_lines_buf<m, true>(a, order, groups, ls.prio, lanes8, s);           \
         else launch_lines_buf<m, false>(a, order, groups, ls.prio, false, s);                      \
         break;
             CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
@@ -291,6 +345,28 @@ int launch_bt_planes(const uint8_t *L, const uint8_t *R, int W, int H, int pitch
     prof_begin(s);
     bt_planes_kernel<<<grid, 256, 0, s>>>(L, R, W, H, pitch, m, LT, RT, Wr);
     return check_launch("bt_planes_kernel", s);
+}
+
+// Relative cost bytes a.CR of every pixel from the term planes (fused) or
+// the SPLIT16 volume a.C.
+int launch_sgm_rel(const SgmArgs &a, hipStream_t s)
+{
+    const bool fused = a.C == nullptr;
+    prof_begin(s);
+    switch (a.m) {
+#define CASE(m)                                                                                    \
+    case m: {                                                                                      \
+        const long threads = (long)a.W * a.H * sum_group<m>();                                    \
+        const int blocks = (int)((threads + 255) / 256);                                           \
+        if (fused) sgm_rel_kernel<m, true><<<blocks, 256, 0, s>>>(a);                              \
+        else sgm_rel_kernel<m, false><<<blocks, 256, 0, s>>>(a);                                   \
+        break;                                                                                     \
+    }
+        CASE(1) CASE(2) CASE(4) CASE(6) CASE(8)
+#undef CASE
+    default: set_error("no relative cost bytes at %d words per lane", a.m); return TDVH_ERR_UNSUPPORTED;
+    }
+    return check_launch("sgm_rel_kernel", s);
 }
 
 // Wide-lane plane sum (16 bytes per lane and plane) for power-of-two word
@@ -438,6 +514,18 @@ static int sgm_std(tdvh_ctx *ctx, const uint32_t *Csplit, const uint8_t *left,
             TDVH_TRY(launch_bt_terms(left, right, W, H, pitch, TL, TR, s));
             TDVH_TRY(launch_bt_split(TL, TR, W, H, D, (uint32_t *)cb, s));
             a.C = (const uint32_t *)cb;
+        }
+    }
+    // relative cost bytes (byte planes, 2 P2 below 128, an even word count,
+    // buffer-addressed scans)
+    if (eout && sgm_rel_auto(W, H, m, params->p2) && lines_buf_enabled()) {
+        SgmArgs r = a;
+        r.CR = reinterpret_cast<uint8_t *>(1); // sized below; checked by buf_lines_fit
+        if (buf_lines_fit(r)) {
+            void *cr = nullptr;
+            TDVH_TRY(ctx->scratch("sgm_rel", (size_t)W * H * 32 * m, &cr));
+            a.CR = (uint8_t *)cr;
+            TDVH_TRY(launch_sgm_rel(a, s));
         }
     }
     TDVH_TRY(launch_sgm_lines(ctx, a, params->dir_mask, eout, s));

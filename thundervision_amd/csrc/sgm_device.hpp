@@ -53,7 +53,14 @@ struct SgmArgs {
     uint8_t *planes;    // path-cost planes (E bytes or L words)
     size_t plane_bytes;
     uint8_t *trash;     // >= 64 * 64 bytes: stores of lanes past their line
+    uint8_t *CR;        // relative cost bytes (sgm_rel_pixel), or null
 };
+
+// Cost sources of the buffer line scans: a SPLIT16 uint16 volume, BT
+// recomputed from the term planes, or the relative cost bytes.
+constexpr int SRC_WORDS = 0;
+constexpr int SRC_TERMS = 1;
+constexpr int SRC_REL = 2;
 
 // Packed uint16 saturating subtract / max.
 __device__ __forceinline__ uint32_t pk_subs(uint32_t a, uint32_t b)
@@ -191,6 +198,21 @@ __device__ __forceinline__ void sgm_step(uint32_t (&Lp)[M], const uint32_t (&c)[
 __device__ __forceinline__ uint32_t pack_e(uint32_t a, uint32_t b)
 {
     return __builtin_amdgcn_perm(b, a, 0x06040200u);
+}
+
+// Unpacks the two bytes of half h of word w into a packed uint16x2.
+__device__ __forceinline__ uint32_t unpack_e(uint32_t w, int h)
+{
+    return __builtin_amdgcn_perm(0u, w, h ? 0x0C030C02u : 0x0C010C00u);
+}
+
+// Unpacks two relative cost bytes (half h of word w) into a packed
+// uint16x2, the high byte of each half copied from the sign of the odd
+// byte of the pair (v_perm_b32 selectors 8 and 9): values below 128 stay,
+// and a pixel marked 0xFF in every byte (no valid cell) reads 0xFFFF.
+__device__ __forceinline__ uint32_t unpack_rel(uint32_t w, int h)
+{
+    return __builtin_amdgcn_perm(w, w, h ? 0x09030902u : 0x08010800u);
 }
 
 // One wave scans lines line0 .. line0+3 of direction `dir` and writes its
@@ -362,12 +384,17 @@ __device__ __forceinline__ void buf_store(buf_t r, uint32_t off, const uint32_t 
 // the lanes owning each word differ). A smaller word count per lane leaves
 // the step's fixed costs (loads, stores, left terms, the group minimum)
 // spread over few cells; 8-lane lines halve the waves for the same work.
-template <int M, bool FUSED, bool PAD, int PF, int LPL = 16>
+// SRC: where the step's costs come from (SRC_WORDS, SRC_TERMS, SRC_REL).
+template <int M, int SRC, bool PAD, int PF, int LPL = 16>
 __device__ __forceinline__ void sgm_line_group_buf(const SgmArgs &a, int dir, int line0, int s,
                                                    int prio)
 {
-    constexpr int MW = M * 16 / LPL;           // words per lane and plane
-    constexpr int NW = FUSED ? 1 + 3 * MW : MW; // prefetched words per step
+    constexpr bool FUSED = SRC == SRC_TERMS;
+    constexpr bool REL = SRC == SRC_REL;
+    constexpr int MW = M * 16 / LPL; // words per lane and plane
+    static_assert(!REL || MW == 1 || MW % 2 == 0, "relative cost bytes: 1 or an even word count");
+    // prefetched words per step
+    constexpr int NW = FUSED ? 1 + 3 * MW : (REL ? (MW + 1) / 2 : MW);
     const int lane = threadIdx.x & 63;
     const int line = line0 + lane / LPL;
     const int j = lane % LPL;
@@ -401,6 +428,11 @@ __device__ __forceinline__ void sgm_line_group_buf(const SgmArgs &a, int dir, in
         da = (uint32_t)(step * 4);
         fb = (uint32_t)(sy * a.Wr + (W - 1 - sx) + j * MW) * 4;
         db = (uint32_t)((dy * a.Wr - dx) * 4);
+    } else if constexpr (REL) {
+        // 2 bytes per word, in the byte order of the E planes
+        src[0] = buf_rsrc(a.CR, (uint32_t)W * H * (32 * M));
+        fa = (uint32_t)p0 * (32 * M) + j * (2 * MW);
+        da = (uint32_t)(step * (32 * M));
     } else {
         src[0] = buf_rsrc(a.C, (uint32_t)W * H * (64 * M));
         fa = (uint32_t)p0 * (64 * M) + j * (4 * MW);
@@ -414,8 +446,10 @@ __device__ __forceinline__ void sgm_line_group_buf(const SgmArgs &a, int dir, in
 #pragma unroll
             for (int t = 0; t < 3; t++) buf_load<MW>(src[1 + t], fb, slot + 1 + t * MW);
             fb += db;
+        } else if constexpr (REL && MW == 1) {
+            slot[0] = __builtin_amdgcn_raw_buffer_load_b16(src[0], fa, 0, 0);
         } else {
-            buf_load<MW>(src[0], fa, slot);
+            buf_load<NW>(src[0], fa, slot);
         }
         fa += da;
     };
@@ -454,6 +488,9 @@ __device__ __forceinline__ void sgm_line_group_buf(const SgmArgs &a, int dir, in
             for (int r = 0; r < MW; r++) s3[r] = pk_max(s3[r], s4[r]);
 #pragma unroll
             for (int r = 0; r < MW; r++) c[r] = pk_min(s1[r], s3[r]);
+        } else if constexpr (REL) {
+#pragma unroll
+            for (int r = 0; r < MW; r++) c[r] = unpack_rel(slot[r / 2], r % 2);
         } else {
 #pragma unroll
             for (int r = 0; r < MW; r++) c[r] = slot[r];
@@ -526,13 +563,24 @@ template <int M, bool FUSED> constexpr int buf_lines_pf()
 // soon as any wave ends; `order` gives workgroup b its group (entry: plane
 // slot << 28 | direction << 24 | first line; 64 / LPL lines from there),
 // longest first within each chiplet's share.
-template <int M, bool FUSED, bool PAD, int LPL>
+template <int M, int SRC, bool PAD, int LPL>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void
 sgm_lines_buf_kernel(const SgmArgs a, const uint32_t *__restrict__ order, int prio)
 {
     const uint32_t e = order[blockIdx.x];
-    sgm_line_group_buf<M, FUSED, PAD, buf_lines_pf<M, FUSED>(), LPL>(
+    sgm_line_group_buf<M, SRC, PAD, buf_lines_pf<M, SRC == SRC_TERMS>(), LPL>(
         a, (int)((e >> 24) & 0xF), (int)(e & 0xFFFFFF), (int)(e >> 28), prio);
+}
+
+// The line scans over the relative cost bytes: a few words per step, so a
+// deeper prefetch ring than the BT scans (16 steps) fits the registers.
+template <int M, int LPL>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void
+sgm_lines_rel_kernel(const SgmArgs a, const uint32_t *__restrict__ order, int prio)
+{
+    const uint32_t e = order[blockIdx.x];
+    sgm_line_group_buf<M, SRC_REL, false, 16, LPL>(a, (int)((e >> 24) & 0xF),
+                                                   (int)(e & 0xFFFFFF), (int)(e >> 28), prio);
 }
 
 // Whether sgm_line_group_buf can address every plane of a (byte planes).
@@ -540,6 +588,7 @@ __host__ __device__ inline bool buf_lines_fit(const SgmArgs &a)
 {
     const size_t lim = (size_t)1 << 31;
     if (a.plane_bytes >= lim) return false;
+    if (a.CR) return (size_t)a.W * a.H * 32 * (size_t)a.m < lim;
     if (a.C) return (size_t)a.W * a.H * 64 * (size_t)a.m < lim;
     return (size_t)a.W * a.H * 4 < lim && (size_t)a.Wr * a.H * 4 < lim;
 }
@@ -598,12 +647,6 @@ __device__ __forceinline__ void bt_planes_at(const uint8_t *__restrict__ L,
 
 
 // ---- plane sum + lowest-index WTA ----------------------------------------
-
-// Unpacks the two bytes of half h of word w into a packed uint16x2.
-__device__ __forceinline__ uint32_t unpack_e(uint32_t w, int h)
-{
-    return __builtin_amdgcn_perm(0u, w, h ? 0x0C030C02u : 0x0C010C00u);
-}
 
 // Sum of the planes and lowest-index argmin of pixel p by one 16-lane row.
 template <int M, bool FUSED, bool EOUT>
@@ -794,6 +837,96 @@ __global__ __launch_bounds__(256) void sgm_sum_wide_kernel(const SgmArgs a, int 
     const long p = ((long)blockIdx.x * 256 + threadIdx.x) / G;
     if (p >= (long)a.W * a.H) return; // whole groups leave together (256 % G == 0)
     sgm_sum_pixel_wide<M, FUSED>(a, nplanes, S, disp, pitch, p, threadIdx.x % G);
+}
+
+// ---- relative cost bytes ---------------------------------------------------
+//
+// The line scans need a pixel's costs only up to a constant per pixel and
+// only below the pixel's minimum path cost plus P2:
+//  * the step takes L' - m, so adding a constant to a pixel's whole vector
+//    changes no increment E = L - C downstream;
+//  * a cell with L'(d) >= m + P2 never changes min(L'(d), L'(d+-1) + P1,
+//    m + P2), so any value >= m + P2 there gives the same E;
+//  * m <= Cmin + P2, Cmin being the pixel's least cost (the cell holding it
+//    has L = Cmin + E with E <= P2), and the cell holding m has C <= m.
+// So CR(d) = min(C(d) - Cmin, 2 P2) gives, in every direction, exactly the
+// increments E of the true costs: cells with C > Cmin + 2 P2 stay at or
+// above m + P2 with either cost, every other cell is shifted by Cmin, and
+// invalid cells (0xFFFF, or >= 0xFE01 from the term planes' x < 0 words)
+// become 2 P2, another "at or above m + P2". A pixel with no valid cell
+// (possible in a given volume) is written as 0xFF bytes, which the scans
+// read as 0xFFFF, so the next step restarts the path as with the true
+// costs. For P2 <= 63 every other byte is below 128 (unpack_rel). One byte
+// per cell in the byte order of the E planes, read by the scans instead of
+// recomputing BT; the plane sum still adds n*C with the true C.
+
+template <int M> constexpr bool rel_ok() { return M == 1 || M == 2 || M == 4 || M == 6 || M == 8; }
+
+// Relative cost bytes of pixel p; lane t < 2M of the pixel's group of G
+// lanes owns SPLIT words 8t .. 8t+7, i.e. bytes 16t .. 16t+15 of the pixel.
+template <int M, bool FUSED>
+__device__ __forceinline__ void sgm_rel_pixel(const SgmArgs &a, long p, int t)
+{
+    static_assert(rel_ok<M>(), "word counts 1, 2, 4, 6, 8");
+    constexpr int G = sum_group<M>();
+    const int W = a.W, H = a.H;
+    const int y = (int)(p / W), x = (int)(p - (long)y * W);
+    const int w0 = 8 * t;
+    const bool act = G == 2 * M || t < 2 * M;
+    uint32_t c[8];
+    uint32_t cmin = 0xFFFFu;
+    if (act) {
+        if constexpr (FUSED) {
+            const size_t rt_plane = (size_t)a.Wr * H;
+            uint32_t l2, lmn, lmx;
+            lt_unpack(a.LT[p], l2, lmn, lmx);
+            const uint32_t *rt = a.RT + (size_t)y * a.Wr + (W - 1 - x) + w0;
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const uint32_t r2 = rt[r], rmn = rt[rt_plane + r], rmx = rt[2 * rt_plane + r];
+                const uint32_t aa = pk_max(pk_subs(l2, rmx), pk_subs(rmn, l2));
+                const uint32_t bb = pk_max(pk_subs(r2, lmx), pk_subs(lmn, r2));
+                c[r] = pk_min(aa, bb);
+            }
+        } else {
+            const uint4 *src = (const uint4 *)(a.C + (size_t)p * (16 * M) + w0);
+            const uint4 u0 = src[0], u1 = src[1];
+            c[0] = u0.x, c[1] = u0.y, c[2] = u0.z, c[3] = u0.w;
+            c[4] = u1.x, c[5] = u1.y, c[6] = u1.z, c[7] = u1.w;
+        }
+        if (a.D < 32 * M) {
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const int w = w0 + r;
+                c[r] |= (w >= a.D ? 0x0000FFFFu : 0u) | (16 * M + w >= a.D ? 0xFFFF0000u : 0u);
+            }
+        }
+        uint32_t mw = c[0];
+#pragma unroll
+        for (int r = 1; r < 8; r++) mw = pk_min(mw, c[r]);
+        cmin = min(mw & 0xFFFFu, mw >> 16);
+    }
+    cmin = group_min_u32<G>(cmin);
+    if (act) {
+        // a pixel with no valid cell (cmin 0xFFFF) is marked 0xFF in every byte
+        const uint32_t cm = pk_splat(cmin), cap = pk_splat(2 * (uint32_t)a.P2);
+        const uint32_t none = cmin == 0xFFFFu ? 0xFFFFFFFFu : 0u;
+        uint32_t h[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) h[r] = pk_min(pk_subs(c[r], cm), cap) | none;
+        uint4 *dst = (uint4 *)(a.CR + (size_t)p * (32 * M) + 16 * t);
+        *dst = make_uint4(pack_e(h[0], h[1]), pack_e(h[2], h[3]), pack_e(h[4], h[5]),
+                          pack_e(h[6], h[7]));
+    }
+}
+
+template <int M, bool FUSED>
+__global__ __launch_bounds__(256) void sgm_rel_kernel(const SgmArgs a)
+{
+    constexpr int G = sum_group<M>();
+    const long p = ((long)blockIdx.x * 256 + threadIdx.x) / G;
+    if (p >= (long)a.W * a.H) return; // whole groups leave together (256 % G == 0)
+    sgm_rel_pixel<M, FUSED>(a, p, threadIdx.x % G);
 }
 
 template <int M, bool FUSED, bool EOUT>
