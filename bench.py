@@ -8,6 +8,7 @@ Prints one JSON line.
 
   python bench.py [--steps K] [--warmup W] [--mode fused|staged]
   python bench.py --profile-run          a few plain steps only (for rocprofv3)
+  python bench.py --profile-run --shape 640x480x64   the same at another pair shape
   python bench.py --copy-calibration N   a plain N-byte device copy (PMC scale)
 """
 import argparse
@@ -135,7 +136,10 @@ def bench_batched(tv, synth, torch, ctx, reps, specs=None, workload=MIX_WORKLOAD
 SERVED_POLICIES = {
     # name: (slots, Scheduler keyword arguments; cell budgets in largest frames)
     "plain_per_pair": (1, {}),
-    "served": (4, {"cell_budget": 1.3, "backfill_s": 0.003}),
+    # frames above a tenth of the largest start only on an idle device
+    # (10 s at 600 pairs/s: p99 5.98 ms against 6.12 ms for one slot, where
+    # letting every frame join gave 6.38 ms)
+    "served": (4, {"cell_budget": 1.3, "backfill_s": 0.003, "join_cells": 0.1}),
 }
 
 
@@ -163,7 +167,7 @@ def served_frames(torch, dev, duration, rate, distinct):
     return serve.poisson_schedule(len(shapes), rate, duration), pool
 
 
-def bench_served(torch, tv, duration=5.0, rate=600.0, distinct=24, policies=None):
+def bench_served(torch, tv, duration=10.0, rate=600.0, distinct=24, policies=None):
     """16 streams (4 each at 640x480/D64, 1280x720/D128, 1280x720/D192,
     1920x1080/D256), Poisson arrivals at `rate` pairs/s in total, each stream
     cycling through `distinct` generator frames. Every policy serves the same
@@ -191,8 +195,9 @@ def bench_served(torch, tv, duration=5.0, rate=600.0, distinct=24, policies=None
             frames.append(serve.Frame(t, s, k, W, H, D, (L, R, disp)))
         ex = serve.GpuExecutor(nslots)
         kw = dict(kw)
-        if "cell_budget" in kw:
-            kw["cell_budget"] = kw["cell_budget"] * big
+        for key in ("cell_budget", "join_cells"):
+            if key in kw:
+                kw[key] = kw[key] * big
         sch = serve.Scheduler(ex, nslots, 64 << 30, **kw)
         # every slot runs every frame shape once before the clock starts
         ex.warm(frames)
@@ -262,6 +267,52 @@ def bench_window(tv, synth, torch, ctx, reps=5):
             row["frac_of_hbm_roofline"] = round(cells * bpc / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
         out[name] = row
     return out
+
+
+def cpu_threads(L, R, D, want, threads=8, reps=2):
+    """The oracle's BT + SGM-8 + WTA on `threads` host threads (each oracle
+    call releases the interpreter lock): the cost volume and the plane sum +
+    WTA by bands of rows (both are per-row), the SGM as its 8 direction scans
+    at once. Checked equal to the single-thread pipeline's disparities."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+
+    import oracle
+
+    H = L.shape[0]
+    bands = [(H * i // threads, H * (i + 1) // threads) for i in range(threads)]
+
+    def once(ex):
+        C = np.empty((H, L.shape[1], D), np.uint16)
+
+        def bt(b):
+            C[b[0]:b[1]] = oracle.cost_bt2(L[b[0]:b[1]], R[b[0]:b[1]], D)
+        list(ex.map(bt, bands))
+        lrs = list(ex.map(lambda d: oracle.sgm_lr(C, 8, 32, d), range(8)))
+        out = np.empty(L.shape, np.int16)
+
+        def sum_wta(b):
+            c = C[b[0]:b[1]]
+            S = np.zeros(c.shape, np.uint32)
+            for lr in lrs:
+                S += lr[b[0]:b[1]]
+            S = np.where(c == oracle.INV16, np.uint32(oracle.INV16), S).astype(np.uint16)
+            out[b[0]:b[1]] = oracle.wta(S)
+        list(ex.map(sum_wta, bands))
+        return out
+
+    with ThreadPoolExecutor(threads) as ex:
+        got = once(ex)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            once(ex)
+        sec = (time.perf_counter() - t0) / reps
+    return {"value": round(L.size * D / sec / 1e6, 3), "unit": "MDE/s", "cores": threads,
+            "kind": "port", "bit_exact": bool((got == want).all()),
+            "sample": "%d full 1280x720 D=128 pairs on %d host threads (BT and plane sum + WTA "
+                      "by row bands, the 8 SGM direction scans at once), oracle/tdvref.c, %s"
+                      % (reps, threads, cpu_model())}
 
 
 def bench_plain_384(tv, synth, torch, ctx, with_cpu, reps=50):
@@ -406,6 +457,8 @@ def main():
     ap.add_argument("--window-only", action="store_true",
                     help="run only the 1920x1080 D=256 window-cost / DP / WTA paths (for profilers)")
     ap.add_argument("--copy-calibration", type=int, default=0)
+    ap.add_argument("--shape", default="",
+                    help="WxHxD of the pair for --profile-run (default: the 1280x720x128 workload)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every pair and stream is independent, so N GPUs "
@@ -429,6 +482,12 @@ def main():
         return
 
     spec = synth.PLAIN_720P
+    if args.shape:
+        if not args.profile_run:
+            print("--shape applies to --profile-run only", file=sys.stderr)
+            sys.exit(2)
+        w, h, d = (int(v) for v in args.shape.lower().split("x"))
+        spec = dict(width=w, height=h, dmax=d - 1, nrect=12, depth=d, seed=2)
     W, H, D = spec["width"], spec["height"], spec["depth"]
     L, R, _ = synth.workload_pair(spec)
     dev = torch.device("cuda:0")
@@ -508,7 +567,7 @@ def main():
     # the 384x288, D = 16 pair (seed 1): GPU call, and the oracle per stage
     plain_384 = bench_plain_384(tv, synth, torch, ctx, None if args.no_cpu_baseline else True)
 
-    cpu = None
+    cpu = cpu_mt = None
     if not args.no_cpu_baseline:
         import oracle
         reps = 3
@@ -523,6 +582,7 @@ def main():
         if not (got == want).all():
             print("ERROR: GPU disparities differ from the oracle", file=sys.stderr)
             sys.exit(1)
+        cpu_mt = cpu_threads(L, R, D, want)
     if served is not None and not served["all_frames_exact"]:
         print("ERROR: a served frame differs from the same frame computed alone", file=sys.stderr)
         sys.exit(1)
@@ -569,6 +629,7 @@ def main():
         "window_1080p": window,
         "plain_384": plain_384,
         "cpu_baseline": cpu,
+        "cpu_baseline_8_threads": cpu_mt,
     }
     print(json.dumps(out))
 

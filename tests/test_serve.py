@@ -99,6 +99,38 @@ def test_backfill_passes_a_blocked_head_only_for_a_while():
     assert done[(3, 0)].t_start >= done[(1, 0)].t_start
 
 
+def test_join_cells_runs_large_frames_one_at_a_time_in_arrival_order():
+    # with join_cells only small frames start next to frames in flight:
+    # medium and large frames run alone, in arrival order, as on one slot
+    sched = serve.poisson_schedule(16, 600.0, 0.5)
+    frames = frames_from(sched, serve.SERVED_STREAMS)
+    big = 1920 * 1080 * 256
+    join = 0.1 * big
+    ex = FakeExecutor(rate=2e11)
+    sch = serve.Scheduler(ex, slots=4, budget_bytes=64 << 30, cell_budget=1.3 * big,
+                          backfill_s=0.003, join_cells=join)
+    starts = []
+    launch = ex.launch
+
+    def watch(f, slot):
+        starts.append((f, sch.in_flight_cells - f.cells))  # the others in flight
+        launch(f, slot)
+
+    ex.launch = watch
+    done = sch.run(frames)
+    assert len(done) == len(frames)
+    # every frame above join_cells started on an idle device
+    assert all(inflight == 0 for f, inflight in starts if f.cells > join)
+    # small frames did join frames in flight
+    assert any(inflight > 0 for f, inflight in starts if f.cells <= join)
+    # frames above join_cells started in arrival order
+    large = [f for f, _ in starts if f.cells > join]
+    assert all(a.t_arrival <= b.t_arrival for a, b in zip(large, large[1:]))
+    # and never overlapped one another
+    spans = sorted((f.t_start, f.t_done) for f in done if f.cells > join)
+    assert all(a[1] <= b[0] + 1e-12 for a, b in zip(spans, spans[1:]))
+
+
 def test_gc_is_paused_while_serving_and_restored():
     import gc
     ex = FakeExecutor()
@@ -161,7 +193,8 @@ def test_cfg5_served_mix_frames_identical_alone(tv, dev):
     """The served workload: 16 streams (4 each at 640x480/D64, 1280x720/D128,
     1280x720/D192, 1920x1080/D256), Poisson arrivals at 600 pairs/s in total,
     a fresh frame per arrival (rectangles drifting with the frame index),
-    64 GB budget, 4 slots with the benchmark's cell budget and backfill.
+    64 GB budget, 4 slots with the benchmark's cell budget, backfill and join
+    size.
     Every served disparity map equals the same frame computed alone."""
     import torch
     from thundervision_amd import synth
@@ -176,7 +209,8 @@ def test_cfg5_served_mix_frames_identical_alone(tv, dev):
         frames.append(serve.Frame(t, s, k, W, H, D, (Ld, Rd, disp)))
     big = max(W * H * D for W, H, D in shapes)
     ex = serve.GpuExecutor(slots=4)
-    sch = serve.Scheduler(ex, 4, 64 << 30, cell_budget=1.3 * big, backfill_s=0.003)
+    sch = serve.Scheduler(ex, 4, 64 << 30, cell_budget=1.3 * big, backfill_s=0.003,
+                          join_cells=0.1 * big)
     ex.warm(frames)
     for f in frames:
         f.payload[2].fill_(-1)

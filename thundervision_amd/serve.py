@@ -63,16 +63,21 @@ class Scheduler:
     the device with each other (a frame always starts on an idle device).
     With backfill_s, frames behind a head frame that does not fit yet may
     start first while the head has waited less than backfill_s seconds.
+    With join_cells, only frames of at most that many cells start while
+    other frames are in flight: larger frames then run one after another in
+    arrival order, as on one slot, and small frames join them instead of
+    waiting behind them.
     """
 
     def __init__(self, executor, slots, budget_bytes, cell_budget=None, backfill_s=None,
-                 largest_first=False):
+                 largest_first=False, join_cells=None):
         self.ex = executor
         self.nslots = slots
         self.budget = budget_bytes
         self.cell_budget = cell_budget
         self.backfill_s = backfill_s
         self.largest_first = largest_first
+        self.join_cells = join_cells
         self.in_use = 0
         self.in_flight_cells = 0
         self.peak_in_use = 0
@@ -80,6 +85,8 @@ class Scheduler:
 
     def _fits(self, f, busy):
         if self.in_use + f.footprint > self.budget:
+            return False
+        if busy and self.join_cells is not None and f.cells > self.join_cells:
             return False
         if self.cell_budget is not None and busy:
             return self.in_flight_cells + f.cells <= self.cell_budget
