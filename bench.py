@@ -447,8 +447,10 @@ def bench_boundary(tv, torch, ctx, L, R, D, reps=20):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    # 200 steps (0.1 s of timed work at 720p): over 50 steps after 5 warm-up
+    # steps the host clock read ~4 % above the steady per-step time
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--mode", choices=["fused", "staged"], default="fused")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
@@ -553,8 +555,9 @@ def main():
 
     batched = batched_hd = None
     if not args.no_batched:
-        batched = bench_batched(tv, synth, torch, ctx, max(3, args.steps // 5))
-        batched_hd = bench_batched(tv, synth, torch, ctx, max(3, args.steps // 5),
+        breps = max(3, min(10, args.steps // 5))
+        batched = bench_batched(tv, synth, torch, ctx, breps)
+        batched_hd = bench_batched(tv, synth, torch, ctx, breps,
                                    specs=hd_specs(), workload=HD_WORKLOAD)
 
     served = None
