@@ -243,6 +243,9 @@ def bench_window(tv, synth, torch, ctx, reps=5):
         "ncc_wta": (lambda: tv.wta(tv.cost_ncc(Ld, Rd, D, radius=4, zero_mean=True, ctx=ctx),
                                    ctx=ctx), 8),
         "fused_ssd_wta": (lambda: tv.fast_wta_ssd(Ld, Rd, D, radius=3, ctx=ctx), None),
+        # the SSD -> DP path without the 2.12 GB volume (each DP wave computes
+        # its row's costs); same disparities as ssd_dp
+        "fused_ssd_dp": (lambda: tv.stereo_ssd_dp(Ld, Rd, D, radius=3, p_occ=20, ctx=ctx), None),
         "fused_ncc_wta": (lambda: tv.fast_wta_ncc(Ld, Rd, D, radius=4, zero_mean=True, ctx=ctx),
                           None),
     }

@@ -205,6 +205,18 @@ int tdvh_fast_wta_ssd(tdvh_ctx *ctx, const uint8_t *left,
                       int depth, int radius, int16_t *disp, int disp_pitch,
                       void *stream);
 
+/* SSD window (radius 1..3, as tdvh_cost_ssd) + scanline DP (as tdvh_dp with
+ * occlusion penalty `p_occ` < 2^20 and the TDVH_DP_START_ZERO /
+ * TDVH_DP_REF_BACKTRACK flags) without a cost volume: each row's costs are
+ * computed by the wave that scans the row. The disparities of tdvh_cost_ssd
+ * followed by tdvh_dp. D <= 256, width <= 4096. No counterpart in the
+ * reference, whose SSDDev + DynamicProgDev chain computes the same map
+ * through a volume (ssd.cu:79-114, dynamicprog.cu:106-122). */
+int tdvh_stereo_ssd_dp(tdvh_ctx *ctx, const uint8_t *left, const uint8_t *right,
+                       int width, int height, int pitch, int depth, int radius,
+                       uint32_t p_occ, int flags, int16_t *disp, int disp_pitch,
+                       void *stream);
+
 /* NCC window (zero-mean or plain, as tdvh_cost_ncc) + lowest-index WTA
  * without a cost volume: the disparities of tdvh_cost_ncc followed by
  * tdvh_wta (TDVH_WTA_LOWEST). No counterpart in the reference (it fuses only

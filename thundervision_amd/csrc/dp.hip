@@ -62,6 +62,7 @@ constexpr uint64_t DP_INF = ~0ull;
 constexpr uint32_t DP_COST_LIMIT = 1u << 22; // bounded form: valid costs below this
 constexpr uint32_t DP_PEN_LIMIT = 1u << 20;  // bounded form: penalties below this
 constexpr int DP_MAX_DEPTH = 512;           // 8 disparities per lane
+constexpr int DP_SSD_MAX_WIDTH = 4096;      // fused SSD + DP: 16 bytes of LDS per column
 
 enum { DP_U64 = 0, DP_B32 = 1, DP_F32 = 2 };
 
@@ -575,6 +576,276 @@ static int run_dp_integer(const DpLaunch &a, bool bounded, hipStream_t s)
     return check_launch("dp_fixup_kernel", s);
 }
 
+
+// ---- fused SSD window + DP ----------------------------------------------------
+//
+// The SSD -> DP path without a cost volume: the wave that scans row y
+// computes that row's (2r+1)x(2r+1) SSD costs itself, column by column, as
+// it goes. The 2r+1 image rows around y (clamped) of both images are staged
+// in LDS transposed, 8 bytes per column (row i in byte i, zero above 2r+1),
+// with r + 1 copies of the last column after it, so a column of either
+// image is one 8-byte read. With the column sums
+//   col(u, d) = sum_i (L(cx(u), y_i) - R(cx(u - d), y_i))^2
+//             = sum L^2 + sum R^2 - 2 sum L R     (v_dot4_u32_u8, 4 rows each)
+// the window cost is the running sum S(x, d) = S(x - 1, d) + col(x + r, d) -
+// col(x - r - 1, d), i.e. the sum over u in [x - r, x + r] of col(u, d):
+// the SSD of ssd.cu:15-29 with clamp-to-edge reads, exact in 32 bits (below
+// 49 * 255^2). Lane l holds d = l*K + k. Each step it reads one new right
+// column, cx(x + r - l*K); cell k uses the column read k steps earlier (a
+// 4-slot history), and the column sum leaving the window is the one that
+// entered it 2r + 1 steps earlier (a ring of 2r + 1 sums per cell). The
+// loop is unrolled over lcm(4, 2r + 1) steps, so every history slot and
+// ring entry is a fixed register. The DP is the bounded 32-bit form of
+// dp_kernel (costs below 2^22 by construction, invalid exactly where d > x
+// or d >= D), with the same decision words and backtrack, so the
+// disparities equal those of the SSD volume followed by the DP.
+__host__ __device__ constexpr int dp_ssd_unroll(int rad)
+{
+    return (2 * rad + 1) % 2 == 0 ? 0 : 4 * (2 * rad + 1); // lcm(4, odd) = 4 * odd
+}
+
+template <int K, int RAD>
+__global__ __launch_bounds__(64) void dp_ssd_kernel(const uint8_t *__restrict__ Limg,
+                                                    const uint8_t *__restrict__ Rimg, int W,
+                                                    int H, int pitch, int D, uint32_t P,
+                                                    int flags, uint8_t *__restrict__ path,
+                                                    int16_t *__restrict__ disp, int dpitch)
+{
+    static_assert(K >= 1 && K <= 4 && RAD >= 1 && RAD <= 3, "K <= 4, radius 1..3");
+    constexpr int CPW = dp_cpw(K), WORD_BITS = 2 * K * CPW;
+    static_assert(CPW == 4, "four columns per decision word");
+    constexpr int NR = 2 * RAD + 1, UNR = dp_ssd_unroll(RAD);
+    extern __shared__ uint2 cols_lds[];
+    const int l = threadIdx.x, y = blockIdx.x;
+    const int WP = W + RAD + 1; // staged columns per image
+    uint2 *const lc = cols_lds, *const rc = cols_lds + WP;
+
+    for (int u = l; u < WP; u += 64) {
+        const int uu = min(u, W - 1);
+        uint32_t lw[2] = {0u, 0u}, rw[2] = {0u, 0u};
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            const int yy = min(max(y - RAD + i, 0), H - 1);
+            lw[i >> 2] |= (uint32_t)Limg[(size_t)yy * pitch + uu] << (8 * (i & 3));
+            rw[i >> 2] |= (uint32_t)Rimg[(size_t)yy * pitch + uu] << (8 * (i & 3));
+        }
+        lc[u] = make_uint2(lw[0], lw[1]);
+        rc[u] = make_uint2(rw[0], rw[1]);
+    }
+    __syncthreads();
+
+    auto sq = [](uint2 v) { return __builtin_amdgcn_udot4(v.x, v.x, __builtin_amdgcn_udot4(v.y, v.y, 0u, false), false); };
+    auto dot = [](uint2 a, uint2 b) { return __builtin_amdgcn_udot4(a.x, b.x, __builtin_amdgcn_udot4(a.y, b.y, 0u, false), false); };
+    const int lk = l * K;
+
+    // the window before column 0 (u in [-1 - RAD, RAD - 1]): its column sums
+    // fill the ring (the sum of column u sits at (u - RAD) mod NR, the step
+    // that brings it in) and their total starts S
+    uint32_t S[K], ring[NR][K];
+#pragma unroll
+    for (int k = 0; k < K; k++) S[k] = 0;
+#pragma unroll
+    for (int i = 0; i < NR; i++) {
+        const int u = -1 - RAD + i;
+        const uint2 Lc = lc[max(u, 0)];
+        const uint32_t sl = sq(Lc);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const uint2 Rc = rc[max(u - lk - k, 0)];
+            ring[i][k] = sl + sq(Rc) - 2u * dot(Lc, Rc);
+            S[k] += ring[i][k];
+        }
+    }
+    // right columns entering at the three steps before column 0
+    uint2 hn[4];
+    uint32_t qn[4];
+#pragma unroll
+    for (int j = 1; j <= 3; j++) {
+        hn[(4 - j) & 3] = rc[max(RAD - j - lk, 0)];
+        qn[(4 - j) & 3] = sq(hn[(4 - j) & 3]);
+    }
+    // invalid cells: d > x (and every x for d >= D); past the first 64K
+    // columns only d >= D, a constant mask per cell
+    int thr[K];
+    uint32_t padm[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        thr[k] = lk + k >= D ? 0x7FFFFFFF : lk + k;
+        padm[k] = lk + k >= D ? INV32 : 0u;
+    }
+    uint2 Ln = lc[RAD];
+    hn[0] = rc[max(RAD - lk, 0)];
+    // costs of column x (ri = x mod NR, slot = x mod 4: compile-time in the
+    // unrolled loop), then the reads of column x + 1 (in bounds up to x + 1 =
+    // W thanks to the copies after the last column)
+    auto costs = [&](int x, int ri, int slot, bool edge, uint32_t(&c)[K]) {
+        const uint2 ln = Ln;
+        const uint32_t sln = sq(ln);
+        qn[slot] = sq(hn[slot]);
+        uint32_t dn[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) dn[k] = dot(ln, hn[(slot - k) & 3]);
+        Ln = lc[x + 1 + RAD];
+        hn[(slot + 1) & 3] = rc[max(x + 1 + RAD - lk, 0)];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const uint32_t cn = sln + qn[(slot - k) & 3] - 2u * dn[k];
+            S[k] += cn - ring[ri][k];
+            ring[ri][k] = cn;
+            if (edge) c[k] = x < thr[k] ? INV32 : S[k];
+            else c[k] = S[k] | padm[k];
+        }
+    };
+
+    const uint32_t prow_bytes = (uint32_t)dp_row_bytes(W, K);
+    const buf_t prs = buf_rsrc(path + (size_t)y * prow_bytes, prow_bytes);
+    const uint32_t INF = INV32;
+    uint32_t A[K];
+    {
+        uint32_t c[K];
+        costs(0, 0, 0, true, c);
+#pragma unroll
+        for (int k = 0; k < K; k++) A[k] = c[k];
+    }
+    uint32_t mn = 0, left = INF, right = INF;
+    uint32_t cw = 0x55555555u & ((1u << (2 * K)) - 1); // column 0: "stay"
+    // one DP step at column x >= 1 (the bounded form of dp_kernel's step)
+    auto step = [&](int x, int ri, int slot, bool edge) {
+        uint32_t c[K];
+        costs(x, ri, slot, edge, c);
+        uint32_t Sp[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) Sp[k] = dp_add(A[k], P);
+        left = wave_from_prev(left, Sp[K - 1]);
+        right = wave_from_next(right, Sp[0]);
+        uint32_t An[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const uint32_t c1 = k == 0 ? left : Sp[k - 1];
+            const uint32_t c2 = A[k];
+            const uint32_t c3 = k == K - 1 ? right : Sp[k + 1];
+            const bool q = c2 < c3;
+            const uint32_t m23 = q ? c2 : c3;
+            const bool p = c1 < m23;
+            An[k] = dp_add(p ? c1 : m23, c[k]);
+            cw = push_bit(push_bit(cw, p), q);
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) A[k] = An[k];
+        if (slot == 3) {
+            // subtract the minimum taken 4 columns ago, then take this one's
+            const uint32_t pend = (uint32_t)__builtin_amdgcn_readlane((int)mn, 63);
+#pragma unroll
+            for (int k = 0; k < K; k++) A[k] -= pend;
+            mn = A[0];
+#pragma unroll
+            for (int k = 1; k < K; k++) mn = min(mn, A[k]);
+        } else {
+            mn = wave_min_stage(mn, slot);
+        }
+        if (slot == 3 || x == W - 1) { // a partial last word moves up to the full word's bits
+            __builtin_amdgcn_raw_buffer_store_b32(cw << ((3 - slot) * 2 * K), prs,
+                                                  (uint32_t)(x >> 2) * 256u + l * 4, 0, 0);
+            cw = 0;
+        }
+    };
+    // blocks of UNR steps from x = 1 (so x mod NR and x mod 4 are fixed per
+    // unrolled step), then the last partial block with a guard per step
+    int x0 = 1;
+    for (; x0 + UNR <= W && x0 < 64 * K; x0 += UNR) { // blocks with cells d > x
+#pragma unroll
+        for (int u = 0; u < UNR; u++) step(x0 + u, (1 + u) % NR, (1 + u) & 3, true);
+    }
+    for (; x0 + UNR <= W; x0 += UNR) {
+#pragma unroll
+        for (int u = 0; u < UNR; u++) step(x0 + u, (1 + u) % NR, (1 + u) & 3, false);
+    }
+#pragma unroll
+    for (int u = 0; u < UNR - 1; u++)
+        if (x0 + u < W) step(x0 + u, (1 + u) % NR, (1 + u) & 3, true);
+    if (W == 1) // no step: column 0's word
+        __builtin_amdgcn_raw_buffer_store_b32(cw << (3 * 2 * K), prs, l * 4, 0, 0);
+
+    // start: lowest-index argmin of the last column (or d = 0)
+    int start = 0;
+    if (!(flags & TDVH_DP_START_ZERO)) {
+        uint32_t bv = INF;
+        int bd = 0x7FFFFFFF;
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (A[k] < bv) {
+                bv = A[k];
+                bd = lk + k;
+            }
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint32_t ov = (uint32_t)__shfl((int)bv, l ^ off, 64);
+            const int od = __shfl_xor(bd, off, 64);
+            if (ov < bv || (ov == bv && od < bd)) {
+                bv = ov;
+                bd = od;
+            }
+        }
+        start = bd == 0x7FFFFFFF ? 0 : bd;
+    }
+    // backtrack as dp_kernel's
+    constexpr int WPB = 64 / CPW;
+    const bool refbt = flags & TDVH_DP_REF_BACKTRACK;
+    uint32_t z = (uint32_t)__builtin_amdgcn_readfirstlane(start);
+    int16_t *out = disp + (size_t)y * dpitch;
+    auto fetch = [&](int wb, uint32_t(&wv)[WPB]) {
+#pragma unroll
+        for (int i = 0; i < WPB; i++)
+            wv[i] = __builtin_amdgcn_raw_buffer_load_b32(prs, (uint32_t)(wb + i) * 256u + l * 4, 0, 0);
+    };
+    auto walk = [&](int wb, uint32_t(&wv)[WPB], bool partial) {
+#pragma unroll
+        for (int i = 0; i < WPB; i++) {
+            const uint32_t w = wv[i];
+            const uint32_t np = ~(w >> 1) & 0x55555555u;
+            wv[i] = (np & w) | ((np & ~w) << 1);
+        }
+        uint32_t mine = 0;
+#pragma unroll
+        for (int i = WPB - 1; i >= 0; i--) {
+#pragma unroll
+            for (int b = CPW - 1; b >= 0; b--) {
+                if (partial && (wb + i) * CPW + b > W - 1) continue;
+                const uint32_t word = (uint32_t)__builtin_amdgcn_readlane((int)wv[i], (int)(z / K));
+                const uint32_t m = (word >> (WORD_BITS - 2 - 2 * (K * b + (int)(z % K)))) & 3u;
+                const uint32_t nz = min(z + m - 1, (uint32_t)(D - 1));
+                if (refbt) z = nz;
+                mine = l == CPW * i + b ? z : mine;
+                if (!refbt) z = nz;
+            }
+        }
+        if (CPW * wb + l < W) out[CPW * wb + l] = (int16_t)mine;
+    };
+    uint32_t cur[WPB], nxt[WPB];
+    int wb = ((W - 1) / CPW) & ~(WPB - 1);
+    fetch(wb, cur);
+    if (wb >= WPB) fetch(wb - WPB, nxt);
+    walk(wb, cur, true);
+    for (wb -= WPB; wb >= 0; wb -= WPB) {
+#pragma unroll
+        for (int i = 0; i < WPB; i++) cur[i] = nxt[i];
+        if (wb >= WPB) fetch(wb - WPB, nxt);
+        walk(wb, cur, false);
+    }
+}
+
+template <int K>
+static void launch_dp_ssd_k(const uint8_t *L, const uint8_t *R, int W, int H, int pitch, int D,
+                            int radius, uint32_t P, int flags, uint8_t *path, int16_t *disp,
+                            int dpitch, hipStream_t s)
+{
+    const size_t lds = (size_t)(W + radius + 1) * 16;
+    switch (radius) {
+    case 1: dp_ssd_kernel<K, 1><<<H, 64, lds, s>>>(L, R, W, H, pitch, D, P, flags, path, disp, dpitch); break;
+    case 2: dp_ssd_kernel<K, 2><<<H, 64, lds, s>>>(L, R, W, H, pitch, D, P, flags, path, disp, dpitch); break;
+    default: dp_ssd_kernel<K, 3><<<H, 64, lds, s>>>(L, R, W, H, pitch, D, P, flags, path, disp, dpitch); break;
+    }
+}
+
 } // namespace tdvh
 
 using namespace tdvh;
@@ -662,4 +933,37 @@ extern "C" int tdvh_dp_f32(tdvh_ctx *ctx, const tdvh_dsi *C, float p_occ, int fl
     TDVH_REQUIRE(p_occ >= 0.0f && p_occ < __builtin_inff(), "penalty %g not finite and >= 0",
                  (double)p_occ);
     return dp_run(ctx, C, 0, p_occ, flags, disp, disp_pitch, stream);
+}
+
+
+extern "C" int tdvh_stereo_ssd_dp(tdvh_ctx *ctx, const uint8_t *left, const uint8_t *right,
+                                  int width, int height, int pitch, int depth, int radius,
+                                  uint32_t p_occ, int flags, int16_t *disp, int disp_pitch,
+                                  void *stream)
+{
+    clear_error();
+    TDVH_TRY(use_device(ctx));
+    TDVH_REQUIRE(left && right && disp, "null pointer argument");
+    TDVH_REQUIRE(width > 0 && height > 0 && pitch >= width, "bad image %dx%d pitch %d", width,
+                 height, pitch);
+    TDVH_REQUIRE(width <= DP_SSD_MAX_WIDTH, "width %d above %d (the staged rows fill 64 KiB of LDS)",
+                 width, DP_SSD_MAX_WIDTH);
+    TDVH_REQUIRE(depth > 0 && depth <= 256, "depth %d outside [1, 256]", depth);
+    TDVH_REQUIRE(radius >= 1 && radius <= 3, "window radius %d outside [1, 3]", radius);
+    TDVH_REQUIRE(p_occ < DP_PEN_LIMIT, "p_occ %u not below 2^20", p_occ);
+    TDVH_REQUIRE((flags & ~(TDVH_DP_START_ZERO | TDVH_DP_REF_BACKTRACK)) == 0, "bad flags 0x%x",
+                 flags);
+    TDVH_REQUIRE(disp_pitch >= width, "disp_pitch %d < width %d", disp_pitch, width);
+    hipStream_t s = (hipStream_t)stream;
+    const int K = dp_k(depth);
+    void *pb = nullptr;
+    TDVH_TRY(ctx->scratch("dp_path", dp_row_bytes(width, K) * height, &pb));
+    prof_begin(s);
+    switch (K) {
+    case 1: launch_dp_ssd_k<1>(left, right, width, height, pitch, depth, radius, p_occ, flags, (uint8_t *)pb, disp, disp_pitch, s); break;
+    case 2: launch_dp_ssd_k<2>(left, right, width, height, pitch, depth, radius, p_occ, flags, (uint8_t *)pb, disp, disp_pitch, s); break;
+    case 3: launch_dp_ssd_k<3>(left, right, width, height, pitch, depth, radius, p_occ, flags, (uint8_t *)pb, disp, disp_pitch, s); break;
+    default: launch_dp_ssd_k<4>(left, right, width, height, pitch, depth, radius, p_occ, flags, (uint8_t *)pb, disp, disp_pitch, s); break;
+    }
+    return check_launch("dp_ssd_kernel", s);
 }

@@ -79,6 +79,7 @@ SIGNATURES = {
     "tdvh_dp_f32": (_I, [_VP, ct.POINTER(DSI), ct.c_float, _I, _VP, _I, _VP]),
     "tdvh_fast_wta_ssd": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, _I, _VP, _I, _VP]),
     "tdvh_fast_wta_ncc": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _VP, _I, _VP]),
+    "tdvh_stereo_ssd_dp": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, _I, _U32, _I, _VP, _I, _VP]),
     "tdvh_fast_wta_ncc_noalloc": (_I, [_VP, _VP, _I, _I, _I, _I, _I, _I, _VP, _I, _VP]),
     "tdvh_stereo_bt_sgm": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, ct.POINTER(SgmParams), _I, _VP,
                                 _I, _VP]),
@@ -333,6 +334,17 @@ def fast_wta_ssd(left, right, depth, radius=3, ctx=None, stream=None):
     disp = torch.empty((H, W), dtype=torch.int16, device=left.device)
     _check(lib().tdvh_fast_wta_ssd(_ctx(ctx), _ptr(left), _ptr(right), W, H, pitch, depth,
                                    radius, _ptr(disp), W, _stream(stream)))
+    return disp
+
+
+def stereo_ssd_dp(left, right, depth, radius=3, p_occ=20, flags=0, out=None, ctx=None,
+                  stream=None):
+    """SSD window + scanline DP without a volume: dp(cost_ssd(...), p_occ, flags)."""
+    W, H, pitch = _img_args(left, right)
+    disp = out if out is not None else torch.empty((H, W), dtype=torch.int16,
+                                                   device=left.device)
+    _check(lib().tdvh_stereo_ssd_dp(_ctx(ctx), _ptr(left), _ptr(right), W, H, pitch, depth,
+                                    radius, p_occ, flags, _ptr(disp), W, _stream(stream)))
     return disp
 
 
