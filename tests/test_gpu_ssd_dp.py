@@ -93,7 +93,13 @@ def test_fused_ssd_dp_rejects_bad_arguments(tv, dev, kw, msg):
     assert (got == oracle.dp(oracle.cost_ssd(L, R, 16, 3), 20)).all()
 
 
-def test_fused_ssd_dp_rejects_wide_images(tv, dev):
-    L = torch.zeros((2, 4097), dtype=torch.uint8, device=dev)
+def test_fused_ssd_dp_widest_image(tv, dev):
+    """The widest accepted row (4092 columns: 64 KiB of staged columns with
+    radius 3) against the oracle, and the next width rejected."""
+    L, R = pair(4092, 3, 31, 9)
+    want = oracle.dp(oracle.cost_ssd(L, R, 32, 3), 20)
+    got = tv.stereo_ssd_dp(to_dev(L, dev), to_dev(R, dev), 32).cpu().numpy()
+    assert (got == want).all()
+    W = torch.zeros((2, 4093), dtype=torch.uint8, device=dev)
     with pytest.raises(tv.TdvhError, match="width"):
-        tv.stereo_ssd_dp(L, L, 16)
+        tv.stereo_ssd_dp(W, W, 16)

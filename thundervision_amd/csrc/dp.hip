@@ -62,7 +62,9 @@ constexpr uint64_t DP_INF = ~0ull;
 constexpr uint32_t DP_COST_LIMIT = 1u << 22; // bounded form: valid costs below this
 constexpr uint32_t DP_PEN_LIMIT = 1u << 20;  // bounded form: penalties below this
 constexpr int DP_MAX_DEPTH = 512;           // 8 disparities per lane
-constexpr int DP_SSD_MAX_WIDTH = 4096;      // fused SSD + DP: 16 bytes of LDS per column
+// fused SSD + DP: 16 bytes of LDS per staged column, W + radius + 1 columns
+// within 64 KiB
+constexpr int DP_SSD_MAX_WIDTH = 4092;
 
 enum { DP_U64 = 0, DP_B32 = 1, DP_F32 = 2 };
 
