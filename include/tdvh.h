@@ -209,7 +209,8 @@ int tdvh_fast_wta_ssd(tdvh_ctx *ctx, const uint8_t *left,
  * occlusion penalty `p_occ` < 2^20 and the TDVH_DP_START_ZERO /
  * TDVH_DP_REF_BACKTRACK flags) without a cost volume: each row's costs are
  * computed by the wave that scans the row. The disparities of tdvh_cost_ssd
- * followed by tdvh_dp. D <= 256, width <= 4092. No counterpart in the
+ * followed by tdvh_dp. D <= 256, width <= 10236 (the staged rows, 16 bytes
+ * per column, within a workgroup's 160 KiB of LDS). No counterpart in the
  * reference, whose SSDDev + DynamicProgDev chain computes the same map
  * through a volume (ssd.cu:79-114, dynamicprog.cu:106-122). */
 int tdvh_stereo_ssd_dp(tdvh_ctx *ctx, const uint8_t *left, const uint8_t *right,

@@ -3,7 +3,7 @@ the disparities of the oracle's SSD volume followed by its DP, bit-exact,
 for every register shape (D up to 256), window radius 1..3, both reference
 flags, several occlusion penalties, images narrower than the disparity
 range and pitched images; the full 1920x1080 D = 256 workload against the
-oracle; argument validation."""
+oracle; the widest row the LDS holds; argument validation."""
 import numpy as np
 import pytest
 import torch
@@ -94,12 +94,13 @@ def test_fused_ssd_dp_rejects_bad_arguments(tv, dev, kw, msg):
 
 
 def test_fused_ssd_dp_widest_image(tv, dev):
-    """The widest accepted row (4092 columns: 64 KiB of staged columns with
-    radius 3) against the oracle, and the next width rejected."""
-    L, R = pair(4092, 3, 31, 9)
+    """The widest accepted row (10236 columns: 160 KiB of staged columns
+    with radius 3, a workgroup's LDS) against the oracle, and the next width
+    rejected."""
+    L, R = pair(10236, 3, 31, 9)
     want = oracle.dp(oracle.cost_ssd(L, R, 32, 3), 20)
     got = tv.stereo_ssd_dp(to_dev(L, dev), to_dev(R, dev), 32).cpu().numpy()
     assert (got == want).all()
-    W = torch.zeros((2, 4093), dtype=torch.uint8, device=dev)
+    W = torch.zeros((2, 10237), dtype=torch.uint8, device=dev)
     with pytest.raises(tv.TdvhError, match="width"):
         tv.stereo_ssd_dp(W, W, 16)

@@ -52,6 +52,7 @@
 // block of 64 columns each lane loads its block's decision words together,
 // then the whole wave walks the columns in lockstep with a wave-uniform
 // disparity, reading the decision bits of lane z / K by readlane.
+#include <atomic>
 #include <type_traits>
 
 #include "kernels.hpp"
@@ -63,8 +64,8 @@ constexpr uint32_t DP_COST_LIMIT = 1u << 22; // bounded form: valid costs below 
 constexpr uint32_t DP_PEN_LIMIT = 1u << 20;  // bounded form: penalties below this
 constexpr int DP_MAX_DEPTH = 512;           // 8 disparities per lane
 // fused SSD + DP: 16 bytes of LDS per staged column, W + radius + 1 columns
-// within 64 KiB
-constexpr int DP_SSD_MAX_WIDTH = 4092;
+// within a workgroup's 160 KiB
+constexpr int DP_SSD_MAX_WIDTH = 10236;
 
 enum { DP_U64 = 0, DP_B32 = 1, DP_F32 = 2 };
 
@@ -592,11 +593,11 @@ static int run_dp_integer(const DpLaunch &a, bool bounded, hipStream_t s)
 // the window cost is the running sum S(x, d) = S(x - 1, d) + col(x + r, d) -
 // col(x - r - 1, d), i.e. the sum over u in [x - r, x + r] of col(u, d):
 // the SSD of ssd.cu:15-29 with clamp-to-edge reads, exact in 32 bits (below
-// 49 * 255^2). Lane l holds d = l*K + k. Each step it reads one new right
-// column, cx(x + r - l*K); cell k uses the column read k steps earlier (a
-// 4-slot history), and the column sum leaving the window is the one that
-// entered it 2r + 1 steps earlier (a ring of 2r + 1 sums per cell). The
-// loop is unrolled over lcm(4, 2r + 1) steps, so every history slot and
+// 49 * 255^2). A lane holds the cells d = d0 + k (k < K). Each step it reads
+// one new right column, cx(x + r - d0); cell k uses the column read k steps
+// earlier (a 4-slot history), and the column sum leaving the window is the
+// one that entered it 2r + 1 steps earlier (a ring of 2r + 1 sums per cell).
+// The loop is unrolled over lcm(4, 2r + 1) steps, so every history slot and
 // ring entry is a fixed register. The DP is the bounded 32-bit form of
 // dp_kernel (costs below 2^22 by construction, invalid exactly where d > x
 // or d >= D), with the same decision words and backtrack, so the
@@ -606,23 +607,14 @@ __host__ __device__ constexpr int dp_ssd_unroll(int rad)
     return (2 * rad + 1) % 2 == 0 ? 0 : 4 * (2 * rad + 1); // lcm(4, odd) = 4 * odd
 }
 
-template <int K, int RAD>
-__global__ __launch_bounds__(64) void dp_ssd_kernel(const uint8_t *__restrict__ Limg,
-                                                    const uint8_t *__restrict__ Rimg, int W,
-                                                    int H, int pitch, int D, uint32_t P,
-                                                    int flags, uint8_t *__restrict__ path,
-                                                    int16_t *__restrict__ disp, int dpitch)
+// Stages row y's 2r+1 image rows (clamped) of both images transposed into
+// lc / rc, columns u = t, t + nt, ... (W + r + 1 columns).
+template <int RAD>
+__device__ void ssd_stage_rows(const uint8_t *__restrict__ Limg, const uint8_t *__restrict__ Rimg,
+                               int W, int H, int pitch, int y, uint2 *lc, uint2 *rc, int t, int nt)
 {
-    static_assert(K >= 1 && K <= 4 && RAD >= 1 && RAD <= 3, "K <= 4, radius 1..3");
-    constexpr int CPW = dp_cpw(K), WORD_BITS = 2 * K * CPW;
-    static_assert(CPW == 4, "four columns per decision word");
-    constexpr int NR = 2 * RAD + 1, UNR = dp_ssd_unroll(RAD);
-    extern __shared__ uint2 cols_lds[];
-    const int l = threadIdx.x, y = blockIdx.x;
-    const int WP = W + RAD + 1; // staged columns per image
-    uint2 *const lc = cols_lds, *const rc = cols_lds + WP;
-
-    for (int u = l; u < WP; u += 64) {
+    constexpr int NR = 2 * RAD + 1;
+    for (int u = t; u < W + RAD + 1; u += nt) {
         const int uu = min(u, W - 1);
         uint32_t lw[2] = {0u, 0u}, rw[2] = {0u, 0u};
 #pragma unroll
@@ -634,61 +626,79 @@ __global__ __launch_bounds__(64) void dp_ssd_kernel(const uint8_t *__restrict__ 
         lc[u] = make_uint2(lw[0], lw[1]);
         rc[u] = make_uint2(rw[0], rw[1]);
     }
-    __syncthreads();
+}
 
-    auto sq = [](uint2 v) { return __builtin_amdgcn_udot4(v.x, v.x, __builtin_amdgcn_udot4(v.y, v.y, 0u, false), false); };
-    auto dot = [](uint2 a, uint2 b) { return __builtin_amdgcn_udot4(a.x, b.x, __builtin_amdgcn_udot4(a.y, b.y, 0u, false), false); };
-    const int lk = l * K;
+__device__ __forceinline__ uint32_t ssd_sq(uint2 v)
+{
+    return __builtin_amdgcn_udot4(v.x, v.x, __builtin_amdgcn_udot4(v.y, v.y, 0u, false), false);
+}
+__device__ __forceinline__ uint32_t ssd_dot(uint2 a, uint2 b)
+{
+    return __builtin_amdgcn_udot4(a.x, b.x, __builtin_amdgcn_udot4(a.y, b.y, 0u, false), false);
+}
 
-    // the window before column 0 (u in [-1 - RAD, RAD - 1]): its column sums
-    // fill the ring (the sum of column u sits at (u - RAD) mod NR, the step
-    // that brings it in) and their total starts S
-    uint32_t S[K], ring[NR][K];
+// One lane's window costs along its row: cells d = d0 + k, k < K.
+template <int K, int RAD> struct SsdStream {
+    static constexpr int NR = 2 * RAD + 1;
+    const uint2 *lc, *rc;
+    int d0;
+    uint32_t S[K], ring[NR][K], qn[4], padm[K];
+    uint2 hn[4], Ln;
+    int thr[K];
+
+    // the window before column 0 (u in [-1 - r, r - 1]): its column sums
+    // fill the ring (the sum of column u sits at (u - r) mod (2r + 1), the
+    // step that brings it in) and their total starts S; then the right
+    // columns entering at the three steps before column 0, and column 0's
+    // reads
+    __device__ __forceinline__ void init(const uint2 *lc_, const uint2 *rc_, int d0_, int D)
+    {
+        lc = lc_;
+        rc = rc_;
+        d0 = d0_;
 #pragma unroll
-    for (int k = 0; k < K; k++) S[k] = 0;
+        for (int k = 0; k < K; k++) S[k] = 0;
 #pragma unroll
-    for (int i = 0; i < NR; i++) {
-        const int u = -1 - RAD + i;
-        const uint2 Lc = lc[max(u, 0)];
-        const uint32_t sl = sq(Lc);
+        for (int i = 0; i < NR; i++) {
+            const int u = -1 - RAD + i;
+            const uint2 Lc = lc[max(u, 0)];
+            const uint32_t sl = ssd_sq(Lc);
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const uint2 Rc = rc[max(u - d0 - k, 0)];
+                ring[i][k] = sl + ssd_sq(Rc) - 2u * ssd_dot(Lc, Rc);
+                S[k] += ring[i][k];
+            }
+        }
+#pragma unroll
+        for (int j = 1; j <= 3; j++) {
+            hn[(4 - j) & 3] = rc[max(RAD - j - d0, 0)];
+            qn[(4 - j) & 3] = ssd_sq(hn[(4 - j) & 3]);
+        }
+        // invalid cells: d > x (and every x for d >= D); past the first 64K
+        // columns only d >= D, a constant mask per cell
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            const uint2 Rc = rc[max(u - lk - k, 0)];
-            ring[i][k] = sl + sq(Rc) - 2u * dot(Lc, Rc);
-            S[k] += ring[i][k];
+            thr[k] = d0 + k >= D ? 0x7FFFFFFF : d0 + k;
+            padm[k] = d0 + k >= D ? INV32 : 0u;
         }
+        Ln = lc[RAD];
+        hn[0] = rc[max(RAD - d0, 0)];
     }
-    // right columns entering at the three steps before column 0
-    uint2 hn[4];
-    uint32_t qn[4];
-#pragma unroll
-    for (int j = 1; j <= 3; j++) {
-        hn[(4 - j) & 3] = rc[max(RAD - j - lk, 0)];
-        qn[(4 - j) & 3] = sq(hn[(4 - j) & 3]);
-    }
-    // invalid cells: d > x (and every x for d >= D); past the first 64K
-    // columns only d >= D, a constant mask per cell
-    int thr[K];
-    uint32_t padm[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        thr[k] = lk + k >= D ? 0x7FFFFFFF : lk + k;
-        padm[k] = lk + k >= D ? INV32 : 0u;
-    }
-    uint2 Ln = lc[RAD];
-    hn[0] = rc[max(RAD - lk, 0)];
-    // costs of column x (ri = x mod NR, slot = x mod 4: compile-time in the
-    // unrolled loop), then the reads of column x + 1 (in bounds up to x + 1 =
-    // W thanks to the copies after the last column)
-    auto costs = [&](int x, int ri, int slot, bool edge, uint32_t(&c)[K]) {
+
+    // costs of column x (ri = x mod (2r + 1), slot = x mod 4: compile-time
+    // in the unrolled loop), then the reads of column x + 1 (in bounds up to
+    // x + 1 = W thanks to the copies after the last column)
+    __device__ __forceinline__ void costs(int x, int ri, int slot, bool edge, uint32_t (&c)[K])
+    {
         const uint2 ln = Ln;
-        const uint32_t sln = sq(ln);
-        qn[slot] = sq(hn[slot]);
+        const uint32_t sln = ssd_sq(ln);
+        qn[slot] = ssd_sq(hn[slot]);
         uint32_t dn[K];
 #pragma unroll
-        for (int k = 0; k < K; k++) dn[k] = dot(ln, hn[(slot - k) & 3]);
+        for (int k = 0; k < K; k++) dn[k] = ssd_dot(ln, hn[(slot - k) & 3]);
         Ln = lc[x + 1 + RAD];
-        hn[(slot + 1) & 3] = rc[max(x + 1 + RAD - lk, 0)];
+        hn[(slot + 1) & 3] = rc[max(x + 1 + RAD - d0, 0)];
 #pragma unroll
         for (int k = 0; k < K; k++) {
             const uint32_t cn = sln + qn[(slot - k) & 3] - 2u * dn[k];
@@ -697,24 +707,28 @@ __global__ __launch_bounds__(64) void dp_ssd_kernel(const uint8_t *__restrict__ 
             if (edge) c[k] = x < thr[k] ? INV32 : S[k];
             else c[k] = S[k] | padm[k];
         }
-    };
+    }
+};
 
-    const uint32_t prow_bytes = (uint32_t)dp_row_bytes(W, K);
-    const buf_t prs = buf_rsrc(path + (size_t)y * prow_bytes, prow_bytes);
-    const uint32_t INF = INV32;
-    uint32_t A[K];
+// The bounded DP of one lane's K cells (dp_kernel's step): path sums A, the
+// wave minimum being folded, the neighbours' penalised sums and the decision
+// word of the current 4 columns.
+template <int K> struct DpLane {
+    uint32_t A[K], mn, left, right, cw;
+
+    __device__ __forceinline__ void start(const uint32_t (&c)[K])
     {
-        uint32_t c[K];
-        costs(0, 0, 0, true, c);
 #pragma unroll
         for (int k = 0; k < K; k++) A[k] = c[k];
+        mn = 0;
+        left = right = INV32;
+        cw = 0x55555555u & ((1u << (2 * K)) - 1); // column 0: "stay"
     }
-    uint32_t mn = 0, left = INF, right = INF;
-    uint32_t cw = 0x55555555u & ((1u << (2 * K)) - 1); // column 0: "stay"
-    // one DP step at column x >= 1 (the bounded form of dp_kernel's step)
-    auto step = [&](int x, int ri, int slot, bool edge) {
-        uint32_t c[K];
-        costs(x, ri, slot, edge, c);
+
+    // column x >= 1 (slot = x mod 4): every 4 columns the wave minimum of 4
+    // columns earlier is subtracted
+    __device__ __forceinline__ void step(const uint32_t (&c)[K], uint32_t P, int slot)
+    {
         uint32_t Sp[K];
 #pragma unroll
         for (int k = 0; k < K; k++) Sp[k] = dp_add(A[k], P);
@@ -735,7 +749,6 @@ __global__ __launch_bounds__(64) void dp_ssd_kernel(const uint8_t *__restrict__ 
 #pragma unroll
         for (int k = 0; k < K; k++) A[k] = An[k];
         if (slot == 3) {
-            // subtract the minimum taken 4 columns ago, then take this one's
             const uint32_t pend = (uint32_t)__builtin_amdgcn_readlane((int)mn, 63);
 #pragma unroll
             for (int k = 0; k < K; k++) A[k] -= pend;
@@ -745,55 +758,41 @@ __global__ __launch_bounds__(64) void dp_ssd_kernel(const uint8_t *__restrict__ 
         } else {
             mn = wave_min_stage(mn, slot);
         }
-        if (slot == 3 || x == W - 1) { // a partial last word moves up to the full word's bits
-            __builtin_amdgcn_raw_buffer_store_b32(cw << ((3 - slot) * 2 * K), prs,
-                                                  (uint32_t)(x >> 2) * 256u + l * 4, 0, 0);
-            cw = 0;
-        }
-    };
-    // blocks of UNR steps from x = 1 (so x mod NR and x mod 4 are fixed per
-    // unrolled step), then the last partial block with a guard per step
-    int x0 = 1;
-    for (; x0 + UNR <= W && x0 < 64 * K; x0 += UNR) { // blocks with cells d > x
-#pragma unroll
-        for (int u = 0; u < UNR; u++) step(x0 + u, (1 + u) % NR, (1 + u) & 3, true);
     }
-    for (; x0 + UNR <= W; x0 += UNR) {
-#pragma unroll
-        for (int u = 0; u < UNR; u++) step(x0 + u, (1 + u) % NR, (1 + u) & 3, false);
-    }
-#pragma unroll
-    for (int u = 0; u < UNR - 1; u++)
-        if (x0 + u < W) step(x0 + u, (1 + u) % NR, (1 + u) & 3, true);
-    if (W == 1) // no step: column 0's word
-        __builtin_amdgcn_raw_buffer_store_b32(cw << (3 * 2 * K), prs, l * 4, 0, 0);
+};
 
-    // start: lowest-index argmin of the last column (or d = 0)
-    int start = 0;
-    if (!(flags & TDVH_DP_START_ZERO)) {
-        uint32_t bv = INF;
-        int bd = 0x7FFFFFFF;
+// Lowest-index argmin over the wave of the cells d = d0 + k with sum A[k]
+// (every lane gets it; d = 0x7FFFFFFF when no cell is below INV32).
+template <int K>
+__device__ __forceinline__ void wave_best(const uint32_t (&A)[K], int d0, uint32_t &bv, int &bd)
+{
+    bv = INV32;
+    bd = 0x7FFFFFFF;
 #pragma unroll
-        for (int k = 0; k < K; k++)
-            if (A[k] < bv) {
-                bv = A[k];
-                bd = lk + k;
-            }
-        for (int off = 32; off > 0; off >>= 1) {
-            const uint32_t ov = (uint32_t)__shfl((int)bv, l ^ off, 64);
-            const int od = __shfl_xor(bd, off, 64);
-            if (ov < bv || (ov == bv && od < bd)) {
-                bv = ov;
-                bd = od;
-            }
+    for (int k = 0; k < K; k++)
+        if (A[k] < bv) {
+            bv = A[k];
+            bd = d0 + k;
         }
-        start = bd == 0x7FFFFFFF ? 0 : bd;
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t ov = (uint32_t)__shfl_xor((int)bv, off, 64);
+        const int od = __shfl_xor(bd, off, 64);
+        if (ov < bv || (ov == bv && od < bd)) {
+            bv = ov;
+            bd = od;
+        }
     }
-    // backtrack as dp_kernel's
-    constexpr int WPB = 64 / CPW;
-    const bool refbt = flags & TDVH_DP_REF_BACKTRACK;
-    uint32_t z = (uint32_t)__builtin_amdgcn_readfirstlane(start);
-    int16_t *out = disp + (size_t)y * dpitch;
+}
+
+// Backtrack of one row from disparity z over its decision words (per
+// 64-column block each lane loads its words, the next block's while the
+// current one is walked; the wave walks with a wave-uniform z), as
+// dp_kernel's.
+template <int K>
+__device__ __forceinline__ void dp_ssd_backtrack(buf_t prs, int W, int D, bool refbt, uint32_t z,
+                                 int16_t *__restrict__ out, int l)
+{
+    constexpr int CPW = dp_cpw(K), WORD_BITS = 2 * K * CPW, WPB = 64 / CPW;
     auto fetch = [&](int wb, uint32_t(&wv)[WPB]) {
 #pragma unroll
         for (int i = 0; i < WPB; i++)
@@ -833,6 +832,95 @@ __global__ __launch_bounds__(64) void dp_ssd_kernel(const uint8_t *__restrict__ 
         if (wb >= WPB) fetch(wb - WPB, nxt);
         walk(wb, cur, false);
     }
+}
+
+// The forward scan of a row by one wave (lane l: d = l*K + k) from its
+// staged columns, storing the decision words; returns the row's start.
+template <int K, int RAD>
+__device__ __forceinline__ uint32_t dp_ssd_scan(const uint2 *lc, const uint2 *rc, int W, int D,
+                                                uint32_t P, int flags, buf_t prs, int l)
+{
+    static_assert(dp_cpw(K) == 4, "four columns per decision word");
+    constexpr int NR = 2 * RAD + 1, UNR = dp_ssd_unroll(RAD);
+    SsdStream<K, RAD> fs;
+    fs.init(lc, rc, l * K, D);
+    DpLane<K> dl;
+    {
+        uint32_t c[K];
+        fs.costs(0, 0, 0, true, c);
+        dl.start(c);
+    }
+    auto step = [&](int x, int ri, int slot, bool edge) {
+        uint32_t c[K];
+        fs.costs(x, ri, slot, edge, c);
+        dl.step(c, P, slot);
+        if (slot == 3) { // compile-time per unrolled step: no branch in the loop body
+            __builtin_amdgcn_raw_buffer_store_b32(dl.cw, prs, (uint32_t)(x >> 2) * 256u + l * 4, 0, 0);
+            dl.cw = 0;
+        }
+    };
+    // blocks of UNR steps from x = 1 (so x mod NR and x mod 4 are fixed per
+    // unrolled step), then the last partial block with a guard per step
+    int x0 = 1;
+    for (; x0 + UNR <= W && x0 < 64 * K; x0 += UNR) { // blocks with cells d > x
+#pragma unroll
+        for (int u = 0; u < UNR; u++) step(x0 + u, (1 + u) % NR, (1 + u) & 3, true);
+    }
+    for (; x0 + UNR <= W; x0 += UNR) {
+#pragma unroll
+        for (int u = 0; u < UNR; u++) step(x0 + u, (1 + u) % NR, (1 + u) & 3, false);
+    }
+#pragma unroll
+    for (int u = 0; u < UNR - 1; u++)
+        if (x0 + u < W) step(x0 + u, (1 + u) % NR, (1 + u) & 3, true);
+    // a partial last word (column W - 1 not the fourth of its word; W = 1
+    // included) moves up to the full word's bits
+    if (((W - 1) & 3) != 3)
+        __builtin_amdgcn_raw_buffer_store_b32(dl.cw << ((3 - ((W - 1) & 3)) * 2 * K), prs,
+                                              (uint32_t)((W - 1) >> 2) * 256u + l * 4, 0, 0);
+    // start: lowest-index argmin of the last column (or d = 0)
+    if (flags & TDVH_DP_START_ZERO) return 0;
+    uint32_t bv;
+    int bd;
+    wave_best<K>(dl.A, l * K, bv, bd);
+    return bd == 0x7FFFFFFF ? 0 : (uint32_t)bd;
+}
+
+template <int K, int RAD>
+__global__ __launch_bounds__(64) void dp_ssd_kernel(const uint8_t *__restrict__ Limg,
+                                                    const uint8_t *__restrict__ Rimg, int W,
+                                                    int H, int pitch, int D, uint32_t P,
+                                                    int flags, uint8_t *__restrict__ path,
+                                                    int16_t *__restrict__ disp, int dpitch)
+{
+    static_assert(K >= 1 && K <= 4 && RAD >= 1 && RAD <= 3, "K <= 4, radius 1..3");
+    extern __shared__ uint2 cols_lds[];
+    const int l = threadIdx.x, y = blockIdx.x;
+    const int WP = W + RAD + 1; // staged columns per image
+    uint2 *const lc = cols_lds, *const rc = cols_lds + WP;
+    ssd_stage_rows<RAD>(Limg, Rimg, W, H, pitch, y, lc, rc, l, 64);
+    __syncthreads();
+    const uint32_t prow_bytes = (uint32_t)dp_row_bytes(W, K);
+    const buf_t prs = buf_rsrc(path + (size_t)y * prow_bytes, prow_bytes);
+    const uint32_t start = dp_ssd_scan<K, RAD>(lc, rc, W, D, P, flags, prs, l);
+    const uint32_t z = (uint32_t)__builtin_amdgcn_readfirstlane((int)start);
+    dp_ssd_backtrack<K>(prs, W, D, flags & TDVH_DP_REF_BACKTRACK, z, disp + (size_t)y * dpitch, l);
+}
+
+// LDS bytes per workgroup of a device, queried once (64 KiB, what every
+// gfx9 device has, if the query fails).
+static int device_lds_per_workgroup(int dev)
+{
+    static std::atomic<int> cache[64];
+    if (dev < 0 || dev >= 64) return 65536;
+    int v = cache[dev].load(std::memory_order_relaxed);
+    if (v == 0) {
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+            v <= 0)
+            v = 65536;
+        cache[dev].store(v, std::memory_order_relaxed);
+    }
+    return v;
 }
 
 template <int K>
@@ -948,7 +1036,7 @@ extern "C" int tdvh_stereo_ssd_dp(tdvh_ctx *ctx, const uint8_t *left, const uint
     TDVH_REQUIRE(left && right && disp, "null pointer argument");
     TDVH_REQUIRE(width > 0 && height > 0 && pitch >= width, "bad image %dx%d pitch %d", width,
                  height, pitch);
-    TDVH_REQUIRE(width <= DP_SSD_MAX_WIDTH, "width %d above %d (the staged rows fill 64 KiB of LDS)",
+    TDVH_REQUIRE(width <= DP_SSD_MAX_WIDTH, "width %d above %d (the staged rows fill 160 KiB of LDS)",
                  width, DP_SSD_MAX_WIDTH);
     TDVH_REQUIRE(depth > 0 && depth <= 256, "depth %d outside [1, 256]", depth);
     TDVH_REQUIRE(radius >= 1 && radius <= 3, "window radius %d outside [1, 3]", radius);
@@ -960,6 +1048,10 @@ extern "C" int tdvh_stereo_ssd_dp(tdvh_ctx *ctx, const uint8_t *left, const uint
     const int K = dp_k(depth);
     void *pb = nullptr;
     TDVH_TRY(ctx->scratch("dp_path", dp_row_bytes(width, K) * height, &pb));
+    const int max_lds = device_lds_per_workgroup(ctx->device);
+    TDVH_REQUIRE((width + radius + 1) * 16 <= max_lds,
+                 "width %d: %d bytes of staged columns above the device's %d bytes of LDS per workgroup",
+                 width, (width + radius + 1) * 16, max_lds);
     prof_begin(s);
     switch (K) {
     case 1: launch_dp_ssd_k<1>(left, right, width, height, pitch, depth, radius, p_occ, flags, (uint8_t *)pb, disp, disp_pitch, s); break;
