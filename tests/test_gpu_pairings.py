@@ -4,7 +4,13 @@ factory composes (commonstereomatcherfactory.cpp:39-81): SGM over uint32
 bounded 32-bit DP's contract check and 64-bit recomputation, disparity
 ranges above 256 for the costs and the DP, and the float d/D view.
 Integer paths bit-exact; float paths bit-exact against the oracle's
-single-precision restatement with the same operation order."""
+single-precision restatement with the same operation order, both with the
+float path-cost planes summed in direction order (the default) and with one
+launch per direction (forced)."""
+import os
+import subprocess
+import sys
+
 import numpy as np
 import pytest
 import torch
@@ -13,6 +19,8 @@ import oracle
 from thundervision_amd import synth
 
 pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def to_dev(a, dev):
@@ -100,6 +108,55 @@ def test_sgm_f32_nan_inf_and_flat_windows(tv, dev):
     want = oracle.sgm_f32(C, np.float32(0.1), np.float32(0.7))
     assert np.array_equal(S.cpu().numpy().view(np.uint32), want.view(np.uint32))
     assert (disp.cpu().numpy() == oracle.wta(want)).all()
+
+
+_FLOAT_SGM = r"""
+import sys
+sys.path.insert(0, %r)
+import numpy as np, torch
+import oracle
+import thundervision_amd.tdvh as tv
+from thundervision_amd import synth
+dev = torch.device("cuda:0")
+ctx = tv.Context(0)
+names = set()
+for (W, H, D, r, mask) in [(45, 30, 16, 4, 0xFF), (64, 40, 100, 1, 0xFF), (33, 33, 256, 4, 0x3C),
+                           (50, 20, 32, 2, 0x81), (120, 70, 130, 3, 0xFF)]:
+    L, R, _ = synth.pair(W, H, D - 1, 8, 5 * W + r)
+    Ld, Rd = torch.from_numpy(L).to(dev), torch.from_numpy(R).to(dev)
+    C = tv.cost_ncc(Ld, Rd, D, r, True, ctx=ctx)
+    C.view(-1)[:: 97] = float("nan")
+    ctx.profile(True)
+    ctx.profile_reset()
+    disp, S = tv.sgm(C, dir_mask=mask, fp1=0.1, fp2=0.6, return_S=True, ctx=ctx)
+    torch.cuda.synchronize()
+    names |= set(ctx.profile_table())
+    ctx.profile(False)
+    want = oracle.sgm_f32(C.cpu().numpy(), np.float32(0.1), np.float32(0.6), mask)
+    assert np.array_equal(S.cpu().numpy().view(np.uint32), want.view(np.uint32)), (W, H, D, mask)
+    assert (disp.cpu().numpy() == oracle.wta(want)).all()
+print("kernels", " ".join(sorted(names)))
+print("float sgm ok")
+"""
+
+
+@pytest.mark.parametrize("mode", ["planes", "dirs"])
+def test_sgm_f32_both_forms(mode):
+    """SGM over float volumes with the float path-cost planes summed in
+    direction order (the default below 2 GiB planes) and with one launch per
+    direction adding into S (TDVH_SGM_WIDE=dirs, the form of larger
+    volumes), in a child process: the library reads the switch once."""
+    e = dict(os.environ)
+    if mode == "dirs":
+        e["TDVH_SGM_WIDE"] = "dirs"
+    r = subprocess.run([sys.executable, "-c", _FLOAT_SGM % ROOT], env=e, capture_output=True,
+                       text=True, timeout=100)
+    assert r.returncode == 0 and "float sgm ok" in r.stdout, r.stdout + r.stderr
+    kernels = [ln for ln in r.stdout.splitlines() if ln.startswith("kernels ")][0].split()[1:]
+    if mode == "dirs":
+        assert "sgm_wide_dir_kernel" in kernels and "sgm_wide_lines_kernel" not in kernels
+    else:
+        assert "sgm_wide_lines_kernel" in kernels and "sgm_wide_dir_kernel" not in kernels
 
 
 # ---- uint16 SGM: a predecessor with no valid cell restarts the path ----------------

@@ -30,7 +30,14 @@
 // saturates to the same value in any order, and L = C + E saturates only
 // when S does). That moves 5 bytes per cell and direction through the
 // scans instead of 12 and fills the GPU with every direction's lines at
-// once. Other volumes run one launch per direction adding into S.
+// once. Float volumes whose planes stay below 2 GiB run the same way with
+// a float plane of L per direction (the per-direction kernel's operations,
+// cell for cell) and a second kernel that adds the planes in increasing
+// direction index, which gives S the roundings of one launch per direction
+// adding into it. Other volumes run one launch per direction adding into
+// S.
+#include <cstdlib>
+#include <string>
 #include <type_traits>
 
 #include "sgm_device.hpp"
@@ -434,6 +441,209 @@ static int run_wide_e(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params *p
     return check_launch("sgm_wide_sum_kernel", s);
 }
 
+// ---- float volumes, float path-cost planes -----------------------------------------
+
+struct WideFArgs {
+    const uint8_t *C;   // float cost volume
+    int64_t csx, csy;   // its strides in bytes
+    uint8_t *planes;    // one W*H*D float plane per launched direction, [y][x][d]
+    size_t plane_bytes;
+    int W, H, D;
+    float P1, P2;
+    int ndirs;
+    int dir[8];
+    int plane[8]; // plane of each launched direction: its rank by direction index
+    int begin[9]; // first wave of each direction's lines (the last: all waves)
+};
+
+// One wave per line, the lines of every launched direction in one grid
+// (longest directions first), as sgm_wide_e_kernel; each step's L, computed
+// as in sgm_wide_dir_kernel, goes to the direction's plane.
+template <int K, int PF>
+__global__ __launch_bounds__(64) void sgm_wide_l_kernel(const WideFArgs a)
+{
+    typedef WideOps<float> O;
+    const int gw = blockIdx.x;
+    int s = 0;
+    while (s < a.ndirs - 1 && gw >= a.begin[s + 1]) s++; // wave-uniform
+    const int dir = a.dir[s], line = gw - a.begin[s];
+    int sx, sy, len;
+    line_geom(dir, line, a.W, a.H, sx, sy, len);
+    const int dx = dir_dx(dir), dy = dir_dy(dir);
+    const int l = threadIdx.x, d0 = l * K;
+    const uint32_t lane_off = (uint32_t)d0 * 4, cellb = (uint32_t)a.D * 4;
+    const int64_t stepb = dx * a.csx + dy * a.csy;
+    const uint8_t *fp = a.C + sx * a.csx + sy * a.csy;
+    int kf = 0;
+    auto fetch = [&](uint32_t(&slot)[K]) {
+        const buf_t r = buf_rsrc(fp, kf < len ? cellb : 0u);
+        buf_load<K, 0, 2>(r, lane_off, slot); // read once: non-temporal
+        fp += stepb;
+        kf++;
+    };
+    uint32_t ring[PF][K];
+#pragma unroll
+    for (int u = 0; u < PF; u++) fetch(ring[u]);
+
+    const buf_t pr = buf_rsrc(a.planes + (size_t)a.plane[s] * a.plane_bytes, (uint32_t)a.plane_bytes);
+    uint32_t lo = (uint32_t)((((size_t)sy * a.W + sx) * a.D + d0) * 4);
+    const uint32_t dl = (uint32_t)((dy * a.W + dx) * a.D * 4);
+    const int nvalid = a.D - d0; // cells of this lane below D (may be <= 0)
+
+    float Lp[K];
+#pragma unroll
+    for (int q = 0; q < K; q++) Lp[q] = O::inv();
+    float m = O::inv(); // every cell of L' invalid: the next step starts the path
+
+    auto body = [&](uint32_t(&slot)[K]) {
+        float c[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) c[q] = q < nvalid ? O::read(__builtin_bit_cast(float, slot[q])) : O::inv();
+        fetch(slot);
+        float Ln[K];
+        if (!O::valid(m)) { // wave-uniform: path start or restart
+#pragma unroll
+            for (int q = 0; q < K; q++) Ln[q] = c[q];
+        } else {
+            const float left = wide_dpp<0x138>(Lp[K - 1]); // wave_shr:1
+            const float right = wide_dpp<0x130>(Lp[0]);    // wave_shl:1
+#pragma unroll
+            for (int q = 0; q < K; q++) {
+                const float lm = q == 0 ? left : Lp[q - 1];
+                const float lp = q == K - 1 ? right : Lp[q + 1];
+                const float t = O::mn(Lp[q], O::add(O::mn(lm, lp), a.P1));
+                Ln[q] = O::add(c[q], O::excess(t, m, a.P2));
+            }
+        }
+        float mn = Ln[0];
+#pragma unroll
+        for (int q = 1; q < K; q++) mn = O::mn(mn, Ln[q]);
+        m = wave_min(mn);
+        uint32_t w[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            Lp[q] = Ln[q];
+            w[q] = __builtin_bit_cast(uint32_t, Ln[q]);
+        }
+        if (nvalid >= K) {
+            buf_store<K>(pr, lo, w);
+        } else {
+#pragma unroll
+            for (int q = 0; q < K; q++)
+                if (q < nvalid) __builtin_amdgcn_raw_buffer_store_b32(w[q], pr, lo + 4 * q, 0, BUF_STORE_NT);
+        }
+        lo += dl;
+    };
+    int k0 = 0;
+    for (; k0 + PF <= len; k0 += PF) {
+#pragma unroll
+        for (int u = 0; u < PF; u++) body(ring[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < PF; u++)
+        if (k0 + u < len) body(ring[u]);
+}
+
+// S(x, y, d) = L_0 + L_1 + ... over the n planes in order (increasing
+// direction index), one thread per pixel and 4 consecutive d; VEC: D % 4 ==
+// 0 and 16-byte aligned sum cells.
+template <bool VEC>
+__global__ __launch_bounds__(256) void sgm_wide_fsum_kernel(const float *__restrict__ planes,
+                                                            size_t plane_elems, int n,
+                                                            uint8_t *__restrict__ S, int64_t ssx,
+                                                            int64_t ssy, int W, int H, int D)
+{
+    const int G = (D + 3) / 4;
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (int64_t)W * H * G) return;
+    const int g = (int)(idx % G);
+    const int64_t pix = idx / G;
+    const int x = (int)(pix % W), y = (int)(pix / W);
+    const int d = 4 * g;
+    const float *lp = planes + pix * D + d;
+    float *sp = (float *)(S + x * ssx + y * ssy) + d;
+    if constexpr (VEC) {
+        typedef float f32x4 __attribute__((ext_vector_type(4)));
+        f32x4 acc = __builtin_nontemporal_load((const f32x4 *)lp);
+        for (int r = 1; r < n; r++)
+            acc = acc + __builtin_nontemporal_load((const f32x4 *)(lp + (size_t)r * plane_elems));
+        *(f32x4 *)sp = acc;
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (d + q >= D) break;
+            float acc = lp[q];
+            for (int r = 1; r < n; r++) acc = acc + lp[(size_t)r * plane_elems + q];
+            sp[q] = acc;
+        }
+    }
+}
+
+template <int K>
+static void launch_wide_l_k(const WideFArgs &a, hipStream_t s)
+{
+    constexpr int PF = 8;
+    sgm_wide_l_kernel<K, PF><<<a.begin[a.ndirs], 64, 0, s>>>(a);
+}
+
+// The float-plane form for float volumes (planes below 2 GiB).
+static int run_wide_f(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params *p, const tdvh_dsi *S,
+                      hipStream_t s)
+{
+    const int W = C->width, H = C->height, D = C->depth;
+    WideFArgs a{};
+    a.C = (const uint8_t *)C->ptr;
+    a.csx = C->stride_x * 4;
+    a.csy = C->stride_y * 4;
+    a.plane_bytes = (size_t)W * H * D * 4;
+    a.W = W;
+    a.H = H;
+    a.D = D;
+    a.P1 = p->fp1;
+    a.P2 = p->fp2;
+    // longest lines first: horizontal, vertical, then the diagonals
+    int ord[8] = {0, 1, 2, 3, 4, 5, 6, 7};
+    if (W < H) {
+        ord[0] = 2; ord[1] = 3; ord[2] = 0; ord[3] = 1;
+    }
+    a.begin[0] = 0;
+    for (int i = 0; i < 8; i++) {
+        const int dir = ord[i];
+        if (!(p->dir_mask & (1 << dir))) continue;
+        a.dir[a.ndirs] = dir;
+        a.plane[a.ndirs] = popcount8(p->dir_mask & ((1 << dir) - 1));
+        a.begin[a.ndirs + 1] = a.begin[a.ndirs] + dir_lines(dir, W, H);
+        a.ndirs++;
+    }
+    if (a.ndirs == 0) return TDVH_OK;
+    void *pb = nullptr;
+    TDVH_TRY(ctx->scratch("sgm_wide_fplanes", a.plane_bytes * a.ndirs, &pb));
+    a.planes = (uint8_t *)pb;
+    prof_begin(s);
+    switch ((D + 63) / 64) {
+    case 1: launch_wide_l_k<1>(a, s); break;
+    case 2: launch_wide_l_k<2>(a, s); break;
+    case 3: launch_wide_l_k<3>(a, s); break;
+    default: launch_wide_l_k<4>(a, s); break;
+    }
+    TDVH_TRY(check_launch("sgm_wide_lines_kernel", s));
+    const int64_t threads = (int64_t)W * H * ((D + 3) / 4);
+    const unsigned blocks = (unsigned)((threads + 255) / 256);
+    const bool vec = D % 4 == 0 && S->stride_x % 4 == 0 && S->stride_y % 4 == 0 &&
+                     ((uintptr_t)S->ptr & 15) == 0;
+    const size_t pe = (size_t)W * H * D;
+    prof_begin(s);
+    if (vec)
+        sgm_wide_fsum_kernel<true><<<blocks, 256, 0, s>>>((const float *)a.planes, pe, a.ndirs,
+                                                          (uint8_t *)S->ptr, S->stride_x * 4,
+                                                          S->stride_y * 4, W, H, D);
+    else
+        sgm_wide_fsum_kernel<false><<<blocks, 256, 0, s>>>((const float *)a.planes, pe, a.ndirs,
+                                                           (uint8_t *)S->ptr, S->stride_x * 4,
+                                                           S->stride_y * 4, W, H, D);
+    return check_launch("sgm_wide_sum_kernel", s);
+}
+
 template <typename T>
 static int run_wide(const WideArgs<T> &a, int dir_mask, hipStream_t s)
 {
@@ -453,6 +663,17 @@ static int run_wide(const WideArgs<T> &a, int dir_mask, hipStream_t s)
     return TDVH_OK;
 }
 
+// Float planes for the one-launch form: W*H*D floats addressed by 32-bit
+// buffer offsets; TDVH_SGM_WIDE=dirs keeps the launch per direction.
+static bool float_planes_fit(const tdvh_dsi *C)
+{
+    static const bool per_dir = [] {
+        const char *e = getenv("TDVH_SGM_WIDE");
+        return e && std::string(e) == "dirs";
+    }();
+    return !per_dir && (size_t)C->width * C->height * C->depth * 4 < ((size_t)1 << 31);
+}
+
 int run_sgm_wide(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params *p, const tdvh_dsi *S,
                  hipStream_t s)
 {
@@ -465,6 +686,7 @@ int run_sgm_wide(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params *p, con
                              (uint32_t)p->p1,          (uint32_t)p->p2};
         return run_wide(a, p->dir_mask, s);
     }
+    if (float_planes_fit(C)) return run_wide_f(ctx, C, p, S, s);
     WideArgs<float> a{(const float *)C->ptr, C->stride_x, C->stride_y, (float *)S->ptr,
                       S->stride_x,           S->stride_y, C->width,    C->height,
                       C->depth,              p->fp1,      p->fp2};
