@@ -99,7 +99,7 @@ import thundervision_amd.tdvh as tv
 from thundervision_amd import synth
 dev = torch.device("cuda:0")
 for (W, H, D, p2) in [(96, 40, 16, 32), (130, 57, 64, 32), (70, 33, 100, 32), (64, 48, 256, 32),
-                      (80, 30, 48, 300)]:
+                      (80, 30, 48, 300), (150, 44, 192, 32), (90, 37, 150, 32)]:
     L, R, _ = synth.pair(W, H, D - 1, 8, W + D)
     want = oracle.wta(oracle.sgm_std(oracle.cost_bt2(L, R, D), 8, p2))
     Ld, Rd = torch.from_numpy(L).to(dev), torch.from_numpy(R).to(dev)

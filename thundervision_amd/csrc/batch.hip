@@ -134,7 +134,7 @@ batch_lines_buf_kernel(const BPair *__restrict__ pairs, const BItem *__restrict_
                                                  (threadIdx.x >> 6));
     if (w >= n) return;
     const BItem it = items[w];
-    sgm_line_group_buf<M, SRC, PAD, PF, LPL>(pairs[it.pair].a, it.a, it.b, it.a, prio);
+    sgm_line_group_any<M, SRC, PAD, PF, LPL>(pairs[it.pair].a, it.a, it.b, it.a, prio);
 }
 
 // Plane sum + WTA (REL false) or relative cost bytes (REL true) over the
@@ -217,7 +217,7 @@ batch_queue_kernel(const BPair *__restrict__ pairs, const BItem *__restrict__ it
         if (it.kind == IT_BT) {
             run_bt_item(P, it);
         } else if (it.kind == IT_LINES) {
-            sgm_line_group_buf<M, SRC_TERMS, PAD, PF, LPL>(P.a, it.a, it.b, it.a, prio);
+            sgm_line_group_any<M, SRC_TERMS, PAD, PF, LPL>(P.a, it.a, it.b, it.a, prio);
             __builtin_amdgcn_s_setprio(0);
         } else {
             run_sum_units<M>(P, it.a, it.b);

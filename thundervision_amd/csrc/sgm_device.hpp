@@ -385,7 +385,14 @@ __device__ __forceinline__ void buf_store(buf_t r, uint32_t off, const uint32_t 
 // the step's fixed costs (loads, stores, left terms, the group minimum)
 // spread over few cells; 8-lane lines halve the waves for the same work.
 // SRC: where the step's costs come from (SRC_WORDS, SRC_TERMS, SRC_REL).
-template <int M, int SRC, bool PAD, int PF, int LPL = 16>
+// HZ (BT terms, 16-lane lines): 1 for direction (1, 0), 2 for (-1, 0). Along
+// a horizontal line the right-term words of step k + 1 are those of step k
+// moved by one word (lane j's word r is RT word W-1-x + j*M + r of the
+// line's row), so each plane's M words per lane shift by one register, one
+// DPP row shift carries the boundary word between lanes, and only the word
+// entering at the end lane is loaded: 3 dwords per lane and step instead of
+// 3M (the loads, not the arithmetic, bound the scans with BT recomputed).
+template <int M, int SRC, bool PAD, int PF, int LPL = 16, int HZ = 0>
 __device__ __forceinline__ void sgm_line_group_buf(const SgmArgs &a, int dir, int line0, int s,
                                                    int prio)
 {
@@ -393,8 +400,9 @@ __device__ __forceinline__ void sgm_line_group_buf(const SgmArgs &a, int dir, in
     constexpr bool REL = SRC == SRC_REL;
     constexpr int MW = M * 16 / LPL; // words per lane and plane
     static_assert(!REL || MW == 1 || MW % 2 == 0, "relative cost bytes: 1 or an even word count");
+    static_assert(HZ == 0 || (FUSED && LPL == 16), "sliding right terms: BT terms, 16-lane lines");
     // prefetched words per step
-    constexpr int NW = FUSED ? 1 + 3 * MW : (REL ? (MW + 1) / 2 : MW);
+    constexpr int NW = FUSED ? (HZ ? 4 : 1 + 3 * MW) : (REL ? (MW + 1) / 2 : MW);
     const int lane = threadIdx.x & 63;
     const int line = line0 + lane / LPL;
     const int j = lane % LPL;
@@ -438,10 +446,25 @@ __device__ __forceinline__ void sgm_line_group_buf(const SgmArgs &a, int dir, in
         fa = (uint32_t)p0 * (64 * M) + j * (4 * MW);
         da = (uint32_t)(step * (64 * M));
     }
+    // HZ: the right-term words of step 0, and the offset of the word entering
+    // at the end lane (lane 0's word 0 moving right, lane 15's word M - 1
+    // moving left; the same address for the line's 16 lanes)
+    uint32_t rt[3][HZ ? MW : 1];
+    uint32_t fn = 0;
+    if constexpr (HZ != 0) {
+#pragma unroll
+        for (int t = 0; t < 3; t++) buf_load<MW>(src[1 + t], fb, rt[t]);
+        fn = (uint32_t)(sy * a.Wr + (W - 1 - sx) + (HZ == 2 ? 16 * MW - 1 : 0)) * 4;
+    }
 
     uint32_t ring[PF][NW];
     auto fetch = [&](uint32_t(&slot)[NW]) {
-        if constexpr (FUSED) {
+        if constexpr (FUSED && HZ != 0) {
+            slot[0] = __builtin_amdgcn_raw_buffer_load_b32(src[0], fa, 0, 0);
+#pragma unroll
+            for (int t = 0; t < 3; t++) slot[1 + t] = __builtin_amdgcn_raw_buffer_load_b32(src[1 + t], fn, 0, 0);
+            fn += db;
+        } else if constexpr (FUSED) {
             slot[0] = __builtin_amdgcn_raw_buffer_load_b32(src[0], fa, 0, 0);
 #pragma unroll
             for (int t = 0; t < 3; t++) buf_load<MW>(src[1 + t], fb, slot + 1 + t * MW);
@@ -466,22 +489,55 @@ __device__ __forceinline__ void sgm_line_group_buf(const SgmArgs &a, int dir, in
 #pragma unroll
     for (int r = 0; r < MW; r++) padm[r] = cell_mask<MW, LPL>(j, r, a.D, 1 << 30);
 
-    // Costs of one step from a ring slot.
-    auto costs = [&](const uint32_t(&slot)[NW], uint32_t(&c)[MW]) {
+    // Costs of one step from a ring slot (HZ: shift first the right-term
+    // words to the step, the slot holding the entering words; not for step 0).
+    auto costs = [&](const uint32_t(&slot)[NW], uint32_t(&c)[MW], bool shift) {
         if constexpr (FUSED) {
             uint32_t l2, lmn, lmx;
             lt_unpack(slot[0], l2, lmn, lmx);
+            uint32_t rw[3][MW]; // the step's right-term words per plane
+            if constexpr (HZ != 0) {
+                if (shift) {
+#pragma unroll
+                    for (int t = 0; t < 3; t++) {
+                        if constexpr (HZ == 1) {
+                            // lane j's word 0 <- lane j - 1's word M - 1; lane 0 <- the new word
+                            const uint32_t b = (uint32_t)__builtin_amdgcn_update_dpp(
+                                (int)slot[1 + t], (int)rt[t][MW - 1], 0x111, 0xF, 0xF, false);
+#pragma unroll
+                            for (int r = MW - 1; r > 0; r--) rt[t][r] = rt[t][r - 1];
+                            rt[t][0] = b;
+                        } else {
+                            // lane j's word M - 1 <- lane j + 1's word 0; lane 15 <- the new word
+                            const uint32_t b = (uint32_t)__builtin_amdgcn_update_dpp(
+                                (int)slot[1 + t], (int)rt[t][0], 0x101, 0xF, 0xF, false);
+#pragma unroll
+                            for (int r = 0; r < MW - 1; r++) rt[t][r] = rt[t][r + 1];
+                            rt[t][MW - 1] = b;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < 3; t++)
+#pragma unroll
+                    for (int r = 0; r < MW; r++) rw[t][r] = rt[t][r];
+            } else {
+#pragma unroll
+                for (int t = 0; t < 3; t++)
+#pragma unroll
+                    for (int r = 0; r < MW; r++) rw[t][r] = slot[1 + t * MW + r];
+            }
             // stage by stage over the words (independent ops between each
             // packed op and its operands' producers)
             uint32_t s1[MW], s2[MW], s3[MW], s4[MW];
 #pragma unroll
-            for (int r = 0; r < MW; r++) s1[r] = pk_subs(l2, slot[1 + 2 * MW + r]);
+            for (int r = 0; r < MW; r++) s1[r] = pk_subs(l2, rw[2][r]);
 #pragma unroll
-            for (int r = 0; r < MW; r++) s2[r] = pk_subs(slot[1 + MW + r], l2);
+            for (int r = 0; r < MW; r++) s2[r] = pk_subs(rw[1][r], l2);
 #pragma unroll
-            for (int r = 0; r < MW; r++) s3[r] = pk_subs(slot[1 + r], lmx);
+            for (int r = 0; r < MW; r++) s3[r] = pk_subs(rw[0][r], lmx);
 #pragma unroll
-            for (int r = 0; r < MW; r++) s4[r] = pk_subs(lmn, slot[1 + r]);
+            for (int r = 0; r < MW; r++) s4[r] = pk_subs(lmn, rw[0][r]);
 #pragma unroll
             for (int r = 0; r < MW; r++) s1[r] = pk_max(s1[r], s2[r]);
 #pragma unroll
@@ -508,7 +564,7 @@ __device__ __forceinline__ void sgm_line_group_buf(const SgmArgs &a, int dir, in
     // barrier keeps each body's waits on its own slot, so the compiler cannot
     // hoist later steps' cost math (and their waits) above earlier steps.
     uint32_t c[MW];
-    costs(ring[0], c);
+    costs(ring[0], c, false);
     fetch(ring[0]);
     auto body = [&](int u, int k) {
         uint32_t E[MW];
@@ -530,7 +586,7 @@ __device__ __forceinline__ void sgm_line_group_buf(const SgmArgs &a, int dir, in
                 __builtin_amdgcn_raw_buffer_store_b16((unsigned short)pack_e(E[r], 0u), out,
                                                       so + 2 * r, 0, BUF_STORE_NT);
         }
-        costs(ring[(u + 1) % PF], c);
+        costs(ring[(u + 1) % PF], c, true);
         // refill only after the slot's words are consumed, so the loads
         // reuse its registers instead of needing a second copy of the ring
         __builtin_amdgcn_sched_barrier(0);
@@ -559,6 +615,28 @@ template <int M, bool FUSED> constexpr int buf_lines_pf()
     return !FUSED || M <= 4 ? 8 : M == 6 ? 6 : 4;
 }
 
+// A line group of direction `dir`, with the horizontal directions' sliding
+// right terms where they pay: BT terms, 16-lane lines, 5 or 6 words per lane
+// (measured at 1280x720: scans 7 % faster at D = 160, 10 % at D = 192; even
+// or up to 3 % slower at the other word counts, where the loads of the
+// other directions and the horizontal lines' instruction issue set the time).
+template <int M, int SRC, bool PAD, int PF, int LPL>
+__device__ __forceinline__ void sgm_line_group_any(const SgmArgs &a, int dir, int line0, int s,
+                                                   int prio)
+{
+    if constexpr (SRC == SRC_TERMS && LPL == 16 && (M == 5 || M == 6)) {
+        if (dir == 0) {
+            sgm_line_group_buf<M, SRC, PAD, PF, LPL, 1>(a, dir, line0, s, prio);
+            return;
+        }
+        if (dir == 1) {
+            sgm_line_group_buf<M, SRC, PAD, PF, LPL, 2>(a, dir, line0, s, prio);
+            return;
+        }
+    }
+    sgm_line_group_buf<M, SRC, PAD, PF, LPL>(a, dir, line0, s, prio);
+}
+
 // One wave per workgroup, so the dispatcher starts the next line group as
 // soon as any wave ends; `order` gives workgroup b its group (entry: plane
 // slot << 28 | direction << 24 | first line; 64 / LPL lines from there),
@@ -568,7 +646,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void
 sgm_lines_buf_kernel(const SgmArgs a, const uint32_t *__restrict__ order, int prio)
 {
     const uint32_t e = order[blockIdx.x];
-    sgm_line_group_buf<M, SRC, PAD, buf_lines_pf<M, SRC == SRC_TERMS>(), LPL>(
+    sgm_line_group_any<M, SRC, PAD, buf_lines_pf<M, SRC == SRC_TERMS>(), LPL>(
         a, (int)((e >> 24) & 0xF), (int)(e & 0xFFFFFF), (int)(e >> 28), prio);
 }
 
