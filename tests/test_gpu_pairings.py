@@ -80,6 +80,59 @@ def test_sgm_u32_invalid_holes_restarts_and_saturation(tv, dev):
         assert (disp.cpu().numpy() == oracle.wta(want)).all()
 
 
+_U32_SGM = r"""
+import sys
+sys.path.insert(0, %r)
+import numpy as np, torch
+import oracle
+import thundervision_amd.tdvh as tv
+from thundervision_amd import synth
+dev = torch.device("cuda:0")
+ctx = tv.Context(0)
+names = set()
+rng = np.random.default_rng(5)
+for (W, H, D, p1, p2, mask) in [(60, 33, 16, 7, 32, 0xFF), (45, 40, 100, 8, 127, 0xFF),
+                                (33, 21, 130, 3, 0, 0x3C), (70, 25, 256, 20, 64, 0xFF),
+                                (17, 9, 5, 1, 2, 0x81), (90, 30, 190, 8, 32, 0xFF)]:
+    C = rng.integers(0, 1 << 21, (H, W, D)).astype(np.uint32)
+    C[rng.random(C.shape) < 0.5] = rng.integers(0, 300)
+    x = np.arange(W)[None, :, None]
+    C[np.broadcast_to(np.arange(D)[None, None, :] > x, C.shape)] = oracle.INV32
+    C[rng.random(C.shape) < 0.03] = oracle.INV32
+    C[H // 2, W // 3, :] = oracle.INV32
+    print("case", W, H, D, p1, p2, mask, flush=True)
+    ctx.profile(True)
+    ctx.profile_reset()
+    disp, S = tv.sgm(torch.from_numpy(C.view(np.int32)).to(dev), p1=p1, p2=p2, dir_mask=mask,
+                     return_S=True, ctx=ctx)
+    torch.cuda.synchronize()
+    names |= set(ctx.profile_table())
+    ctx.profile(False)
+    want = oracle.sgm_u32(C, p1, p2, mask)
+    assert (S.cpu().numpy().view(np.uint32) == want).all(), (W, H, D, p2, mask)
+    assert (disp.cpu().numpy() == oracle.wta(want)).all()
+print("kernels", " ".join(sorted(names)))
+print("u32 sgm ok")
+"""
+
+
+@pytest.mark.parametrize("mode", ["bytes", "words"])
+def test_sgm_u32_both_forms(mode):
+    """SGM over uint32 volumes with P2 <= 127 with the scans over relative
+    cost bytes (the default) and over the uint32 costs (TDVH_SGM_WIDE_REL=0),
+    in a child process (the library reads the switch once): near-minimum
+    cells with small gaps and costs up to 2^21, edge and scattered invalid
+    cells, a pixel without a valid cell, P2 = 0 .. 127, masks, K = 1 .. 4."""
+    e = dict(os.environ)
+    if mode == "words":
+        e["TDVH_SGM_WIDE_REL"] = "0"
+    r = subprocess.run([sys.executable, "-c", _U32_SGM % ROOT], env=e, capture_output=True,
+                       text=True, timeout=100)
+    assert r.returncode == 0 and "u32 sgm ok" in r.stdout, r.stdout + r.stderr
+    kernels = [ln for ln in r.stdout.splitlines() if ln.startswith("kernels ")][0].split()[1:]
+    assert ("sgm_wide_rel_kernel" in kernels) == (mode == "bytes"), kernels
+
+
 # ---- SGM over float volumes ------------------------------------------------------
 
 @pytest.mark.parametrize("W,H,D,r,zm,mask", [(45, 30, 16, 4, True, 0xFF),

@@ -392,6 +392,34 @@ def test_sgm_relative_costs_give_the_same_increments(P1, P2):
         assert (e_true[valid] >= 0).all() and (e_true[valid] <= P2).all()
 
 
+@pytest.mark.parametrize("P1,P2,mask", [(7, 32, 0xFF), (0, 127, 0xFF), (30, 5, 0x5A), (3, 0, 0x81)])
+def test_sgm_u32_relative_costs_give_the_same_sums(P1, P2, mask):
+    # the uint32 form (SSD costs up to 2^21): the oracle's S over the true
+    # costs equals its S over the relative costs (invalid cells 2 P2,
+    # all-invalid pixels invalid) plus n (C - CR) on every valid cell, i.e.
+    # every direction adds the same increments; all-invalid pixels stay
+    # invalid in both
+    H, W, D = 11, 26, 20
+    C = rng.integers(0, 1 << 21, (H, W, D)).astype(np.uint32)
+    C[rng.random((H, W, D)) < 0.5] = rng.integers(0, 200)  # near-minimum cells, small gaps
+    for x in range(W):
+        C[:, x, x + 1:] = oracle.INV32
+    C[rng.random((H, W, D)) < 0.05] = oracle.INV32
+    C[5, 9, :] = oracle.INV32
+    C[2, 20, :] = oracle.INV32
+    valid = C != oracle.INV32
+    c = C.astype(np.int64)
+    cmin = np.where(valid, c, 1 << 40).min(axis=2, keepdims=True)
+    rel = np.where(valid, np.minimum(c - cmin, 2 * P2), 2 * P2)
+    rel = np.where(valid.any(axis=2, keepdims=True), rel, oracle.INV32).astype(np.uint32)
+    n = bin(mask).count("1")
+    s_true = oracle.sgm_u32(C, P1, P2, mask).astype(np.int64)
+    s_rel = oracle.sgm_u32(rel, P1, P2, mask).astype(np.int64)
+    assert (s_true[valid] == (s_rel + n * (c - rel.astype(np.int64)))[valid]).all()
+    none = ~valid.any(axis=2)
+    assert (s_true[none] == oracle.INV32).all() and (s_rel[none] == oracle.INV32).all()
+
+
 def test_sgm_relative_costs_fail_with_a_tighter_clip():
     # the clip at 2 P2 is needed: clipping one below it changes some
     # increments on random volumes (so the test above can fail)

@@ -30,7 +30,15 @@
 // saturates to the same value in any order, and L = C + E saturates only
 // when S does). That moves 5 bytes per cell and direction through the
 // scans instead of 12 and fills the GPU with every direction's lines at
-// once. Float volumes whose planes stay below 2 GiB run the same way with
+// once. With P2 <= 127 the scans read relative cost bytes instead of the
+// uint32 costs: per pixel min(C - Cmin, 2 P2), 2 P2 on invalid cells, 0xFF
+// on every cell of a pixel with no valid cell (read as invalid, restarting
+// the path as the true costs do). Shifting a pixel's costs by a constant
+// and clipping them at 2 P2 above its minimum changes no increment E (a
+// cell at or above Cmin + 2 P2 stays at or above m + P2 with either cost),
+// so the planes are the same; the sum still adds n times the true C. The
+// scans then read 1 byte per cell and direction instead of 4.
+// Float volumes whose planes stay below 2 GiB run the same way with
 // a float plane of L per direction (the per-direction kernel's operations,
 // cell for cell) and a second kernel that adds the planes in increasing
 // direction index, which gives S the roundings of one launch per direction
@@ -220,6 +228,8 @@ __device__ __forceinline__ uint32_t wave_min_u32_all(uint32_t v)
 struct WideEArgs {
     const uint8_t *C;   // uint32 cost volume
     int64_t csx, csy;   // its strides in bytes
+    const uint8_t *CR;  // relative cost bytes, DP bytes per pixel ([y][x][d]), or null
+    int DP;             // D rounded up to 4
     uint8_t *planes;    // one W*H*D byte plane per launched direction, [y][x][d]
     size_t plane_bytes;
     int W, H, D;
@@ -234,7 +244,7 @@ struct WideEArgs {
 // resource based at the step's pixel (ranged to its D cells, or to nothing
 // past the line), so the lane offsets stay constant and the prefetch needs
 // no clamp.
-template <int K, int PF>
+template <int K, int PF, bool REL>
 __global__ __launch_bounds__(64) void sgm_wide_e_kernel(const WideEArgs a)
 {
     const int gw = blockIdx.x;
@@ -245,13 +255,25 @@ __global__ __launch_bounds__(64) void sgm_wide_e_kernel(const WideEArgs a)
     line_geom(dir, line, a.W, a.H, sx, sy, len);
     const int dx = dir_dx(dir), dy = dir_dy(dir);
     const int l = threadIdx.x, d0 = l * K;
-    const uint32_t lane_off = (uint32_t)d0 * 4, cellb = (uint32_t)a.D * 4;
-    const int64_t stepb = dx * a.csx + dy * a.csy;
-    const uint8_t *fp = a.C + sx * a.csx + sy * a.csy;
+    // REL: K bytes per lane at byte d0 of the pixel's DP bytes (aligned for
+    // K = 1, 2, 4; K = 3 reads bytes)
+    const uint32_t lane_off = REL ? (uint32_t)d0 : (uint32_t)d0 * 4;
+    const uint32_t cellb = REL ? (uint32_t)a.DP : (uint32_t)a.D * 4;
+    const int64_t stepb = REL ? ((int64_t)dy * a.W + dx) * a.DP : dx * a.csx + dy * a.csy;
+    const uint8_t *fp = REL ? a.CR + ((int64_t)sy * a.W + sx) * a.DP : a.C + sx * a.csx + sy * a.csy;
     int kf = 0;
     auto fetch = [&](uint32_t(&slot)[K]) {
         const buf_t r = buf_rsrc(fp, kf < len ? cellb : 0u);
-        buf_load<K, 0, 2>(r, lane_off, slot); // read once: non-temporal
+        if constexpr (!REL) {
+            buf_load<K, 0, 2>(r, lane_off, slot); // read once: non-temporal
+        } else if constexpr (K == 4) {
+            slot[0] = __builtin_amdgcn_raw_buffer_load_b32(r, lane_off, 0, 0);
+        } else if constexpr (K == 2) {
+            slot[0] = __builtin_amdgcn_raw_buffer_load_b16(r, lane_off, 0, 0);
+        } else {
+#pragma unroll
+            for (int q = 0; q < K; q++) slot[q] = __builtin_amdgcn_raw_buffer_load_b8(r, lane_off + q, 0, 0);
+        }
         fp += stepb;
         kf++;
     };
@@ -272,7 +294,14 @@ __global__ __launch_bounds__(64) void sgm_wide_e_kernel(const WideEArgs a)
     auto body = [&](uint32_t(&slot)[K]) {
         uint32_t c[K];
 #pragma unroll
-        for (int q = 0; q < K; q++) c[q] = q < nvalid ? slot[q] : INV32;
+        for (int q = 0; q < K; q++) {
+            if constexpr (REL) {
+                const uint32_t b = (K == 2 || K == 4) ? (slot[0] >> (8 * q)) & 0xFFu : slot[q] & 0xFFu;
+                c[q] = q < nvalid && b != 0xFFu ? b : INV32;
+            } else {
+                c[q] = q < nvalid ? slot[q] : INV32;
+            }
+        }
         fetch(slot);
         uint32_t E[K], Ln[K];
         if (m == INV32) { // wave-uniform: path start or restart
@@ -376,11 +405,63 @@ __global__ __launch_bounds__(256) void sgm_wide_sum_kernel(const uint8_t *__rest
     }
 }
 
+// Relative cost bytes of a uint32 volume (the form above): P threads per
+// pixel (the power of two at or above ceil(D / 4)), 4 cells each (VEC: one
+// 16-byte load, D % 4 == 0 and 16-byte aligned cells); the pixel's minimum
+// by shuffles within its P lanes.
+template <int P, bool VEC>
+__global__ __launch_bounds__(256) void sgm_wide_rel_kernel(const uint8_t *__restrict__ C, int64_t csx,
+                                                           int64_t csy, int W, int H, int D, int DP,
+                                                           uint32_t cap, uint8_t *__restrict__ CR)
+{
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t pix = idx / P;
+    const int t = (int)(idx % P);
+    const bool live = pix < (int64_t)W * H; // every lane joins the shuffles
+    const int x = live ? (int)(pix % W) : 0, y = live ? (int)(pix / W) : 0;
+    const uint32_t *cp = (const uint32_t *)(C + x * csx + y * csy);
+    uint32_t v[4];
+    if (VEC && live && 4 * t < D) {
+        typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+        const u32x4 w4 = __builtin_nontemporal_load((const u32x4 *)(cp + 4 * t));
+        v[0] = w4.x, v[1] = w4.y, v[2] = w4.z, v[3] = w4.w;
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int d = 4 * t + q;
+            v[q] = live && d < D ? cp[d] : INV32;
+        }
+    }
+    uint32_t mn = min(min(v[0], v[1]), min(v[2], v[3]));
+#pragma unroll
+    for (int off = P / 2; off > 0; off >>= 1) mn = min(mn, (uint32_t)__shfl_xor((int)mn, off, 64));
+    uint32_t w = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const uint32_t b = mn == INV32 ? 0xFFu : (v[q] == INV32 ? cap : min(v[q] - mn, cap));
+        w |= b << (8 * q);
+    }
+    if (live && 4 * t < DP) *(uint32_t *)(CR + pix * DP + 4 * t) = w;
+}
+
 template <int K>
 static void launch_wide_e_k(const WideEArgs &a, hipStream_t s)
 {
     constexpr int PF = 8;
-    sgm_wide_e_kernel<K, PF><<<a.begin[a.ndirs], 64, 0, s>>>(a);
+    if (a.CR) sgm_wide_e_kernel<K, PF, true><<<a.begin[a.ndirs], 64, 0, s>>>(a);
+    else sgm_wide_e_kernel<K, PF, false><<<a.begin[a.ndirs], 64, 0, s>>>(a);
+}
+
+// Whether the scans read relative cost bytes: P2 <= 127 (every byte but
+// the marker below 255) and the bytes below 2 GiB; TDVH_SGM_WIDE_REL=0 reads
+// the uint32 costs.
+static bool wide_rel_wanted(int W, int H, int D, uint32_t p2)
+{
+    static const int mode = [] {
+        const char *e = getenv("TDVH_SGM_WIDE_REL");
+        return e ? atoi(e) : 1;
+    }();
+    return mode != 0 && p2 <= 127 && (size_t)W * H * ((D + 3) / 4 * 4) < ((size_t)1 << 31);
 }
 
 // The byte-plane form for uint32 volumes (P2 <= 255, planes below 2 GiB).
@@ -416,6 +497,30 @@ static int run_wide_e(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params *p
     void *pb = nullptr;
     TDVH_TRY(ctx->scratch("sgm_wide_planes", a.plane_bytes * a.ndirs, &pb));
     a.planes = (uint8_t *)pb;
+    a.DP = (D + 3) / 4 * 4;
+    if (wide_rel_wanted(W, H, D, a.P2)) {
+        void *cr = nullptr;
+        TDVH_TRY(ctx->scratch("sgm_wide_rel", (size_t)W * H * a.DP, &cr));
+        a.CR = (const uint8_t *)cr;
+        const int g = (D + 3) / 4;
+        const int P = g <= 1 ? 1 : g <= 2 ? 2 : g <= 4 ? 4 : g <= 8 ? 8 : g <= 16 ? 16 : g <= 32 ? 32 : 64;
+        const int64_t threads = (int64_t)W * H * P;
+        const unsigned blocks = (unsigned)((threads + 255) / 256);
+        const uint32_t cap = 2 * a.P2;
+        prof_begin(s);
+        const bool vec = D % 4 == 0 && a.csx % 16 == 0 && a.csy % 16 == 0 &&
+                         ((uintptr_t)C->ptr & 15) == 0;
+        switch (P) {
+#define CASE(p)                                                                                    \
+    case p:                                                                                        \
+        if (vec) sgm_wide_rel_kernel<p, true><<<blocks, 256, 0, s>>>(a.C, a.csx, a.csy, W, H, D, a.DP, cap, (uint8_t *)cr); \
+        else sgm_wide_rel_kernel<p, false><<<blocks, 256, 0, s>>>(a.C, a.csx, a.csy, W, H, D, a.DP, cap, (uint8_t *)cr); \
+        break;
+            CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32) CASE(64)
+#undef CASE
+        }
+        TDVH_TRY(check_launch("sgm_wide_rel_kernel", s));
+    }
     prof_begin(s);
     switch ((D + 63) / 64) {
     case 1: launch_wide_e_k<1>(a, s); break;
