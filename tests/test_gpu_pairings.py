@@ -63,6 +63,7 @@ def test_sgm_u32_reference_strides(tv, dev, W, H, D, p2):
     want = oracle.sgm_u32(C, 7, p2)
     assert (S.cpu().numpy().view(np.uint32) == want).all()
     assert (disp.cpu().numpy() == oracle.wta(want)).all()
+    assert (tv.sgm(Cx, p1=7, p2=p2).cpu().numpy() == oracle.wta(want)).all()
 
 
 def test_sgm_u32_invalid_holes_restarts_and_saturation(tv, dev):
@@ -111,6 +112,14 @@ for (W, H, D, p1, p2, mask) in [(60, 33, 16, 7, 32, 0xFF), (45, 40, 100, 8, 127,
     want = oracle.sgm_u32(C, p1, p2, mask)
     assert (S.cpu().numpy().view(np.uint32) == want).all(), (W, H, D, p2, mask)
     assert (disp.cpu().numpy() == oracle.wta(want)).all()
+    # disparities only: the sum feeds the WTA without writing S
+    ctx.profile(True)
+    ctx.profile_reset()
+    d_only = tv.sgm(torch.from_numpy(C.view(np.int32)).to(dev), p1=p1, p2=p2, dir_mask=mask, ctx=ctx)
+    torch.cuda.synchronize()
+    names |= set(ctx.profile_table())
+    ctx.profile(False)
+    assert (d_only.cpu().numpy() == oracle.wta(want)).all(), (W, H, D, p2, mask)
 print("kernels", " ".join(sorted(names)))
 print("u32 sgm ok")
 """
@@ -131,6 +140,7 @@ def test_sgm_u32_both_forms(mode):
     assert r.returncode == 0 and "u32 sgm ok" in r.stdout, r.stdout + r.stderr
     kernels = [ln for ln in r.stdout.splitlines() if ln.startswith("kernels ")][0].split()[1:]
     assert ("sgm_wide_rel_kernel" in kernels) == (mode == "bytes"), kernels
+    assert "sgm_wide_sum_wta_kernel" in kernels and "sgm_wide_sum_kernel" in kernels, kernels
 
 
 # ---- SGM over float volumes ------------------------------------------------------
@@ -161,6 +171,7 @@ def test_sgm_f32_nan_inf_and_flat_windows(tv, dev):
     want = oracle.sgm_f32(C, np.float32(0.1), np.float32(0.7))
     assert np.array_equal(S.cpu().numpy().view(np.uint32), want.view(np.uint32))
     assert (disp.cpu().numpy() == oracle.wta(want)).all()
+    assert (tv.sgm(to_dev(C, dev), fp1=0.1, fp2=0.7).cpu().numpy() == oracle.wta(want)).all()
 
 
 _FLOAT_SGM = r"""
@@ -188,6 +199,13 @@ for (W, H, D, r, mask) in [(45, 30, 16, 4, 0xFF), (64, 40, 100, 1, 0xFF), (33, 3
     want = oracle.sgm_f32(C.cpu().numpy(), np.float32(0.1), np.float32(0.6), mask)
     assert np.array_equal(S.cpu().numpy().view(np.uint32), want.view(np.uint32)), (W, H, D, mask)
     assert (disp.cpu().numpy() == oracle.wta(want)).all()
+    ctx.profile(True)
+    ctx.profile_reset()
+    d_only = tv.sgm(C, dir_mask=mask, fp1=0.1, fp2=0.6, ctx=ctx)
+    torch.cuda.synchronize()
+    names |= set(ctx.profile_table())
+    ctx.profile(False)
+    assert (d_only.cpu().numpy() == oracle.wta(want)).all(), (W, H, D, mask)
 print("kernels", " ".join(sorted(names)))
 print("float sgm ok")
 """
@@ -208,8 +226,10 @@ def test_sgm_f32_both_forms(mode):
     kernels = [ln for ln in r.stdout.splitlines() if ln.startswith("kernels ")][0].split()[1:]
     if mode == "dirs":
         assert "sgm_wide_dir_kernel" in kernels and "sgm_wide_lines_kernel" not in kernels
+        assert "sgm_wide_sum_wta_kernel" not in kernels
     else:
         assert "sgm_wide_lines_kernel" in kernels and "sgm_wide_dir_kernel" not in kernels
+        assert "sgm_wide_sum_wta_kernel" in kernels
 
 
 # ---- uint16 SGM: a predecessor with no valid cell restarts the path ----------------

@@ -23,8 +23,11 @@ int run_sgm_ref(tdvh_ctx *ctx, const uint32_t *C_split, int W, int H, int D, int
 
 // SGM over a strided uint32 or float volume into S (same dtype, strided),
 // directions of p->dir_mask in increasing index (sgm_wide.hip).
+// SGM over uint32 / float volumes: S into *S, or (S null, one-launch forms
+// only) the lowest-index WTA's disparities straight from the sum.
+bool sgm_wide_one_launch(const tdvh_dsi *C, const tdvh_sgm_params *p);
 int run_sgm_wide(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params *p, const tdvh_dsi *S,
-                 hipStream_t s);
+                 int16_t *disp, int disp_pitch, hipStream_t s);
 
 int popcount8(int mask);
 

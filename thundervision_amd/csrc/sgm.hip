@@ -450,6 +450,8 @@ static int sgm_wide(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params *par
     const int W = C->width, H = C->height, D = C->depth;
     TDVH_REQUIRE(C->layout == TDVH_LAYOUT_STRIDED, "uint32 and float volumes must be strided");
     TDVH_REQUIRE(params->flags == 0, "reference geometry needs a uint16 cost volume");
+    if (!S_out && sgm_wide_one_launch(C, params))
+        return run_sgm_wide(ctx, C, params, nullptr, disp, disp_pitch, (hipStream_t)stream);
     tdvh_dsi S;
     if (S_out) {
         TDVH_REQUIRE(S_out->ptr && S_out->width == W && S_out->height == H &&
@@ -462,7 +464,7 @@ static int sgm_wide(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params *par
         TDVH_TRY(ctx->scratch("sgm_wide_s", (size_t)W * H * D * 4, &sb));
         TDVH_TRY(tdvh_dsi_init(&S, sb, W, H, D, C->dtype, TDVH_LAYOUT_STRIDED));
     }
-    TDVH_TRY(run_sgm_wide(ctx, C, params, &S, (hipStream_t)stream));
+    TDVH_TRY(run_sgm_wide(ctx, C, params, &S, nullptr, 0, (hipStream_t)stream));
     return tdvh_wta(ctx, &S, TDVH_WTA_LOWEST, disp, disp_pitch, stream);
 }
 

@@ -405,6 +405,94 @@ __global__ __launch_bounds__(256) void sgm_wide_sum_kernel(const uint8_t *__rest
     }
 }
 
+// Order-preserving keys of the lowest-index WTA over a sum: uint32 as is
+// (the invalid value all ones), float non-finite sums all ones.
+__device__ __forceinline__ uint32_t wide_key(uint32_t v) { return v; }
+__device__ __forceinline__ uint32_t wide_key(float v)
+{
+    if (!(v < __builtin_inff())) return 0xFFFFFFFFu;
+    const uint32_t b = __builtin_bit_cast(uint32_t, v);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+// The lowest index holding the minimum key over a pixel's P lanes (each lane
+// keeps its first minimum over increasing d; ties between lanes go to the
+// lower d), or 0 when no cell is below the invalid key: wtacpu.cpp:20-33.
+template <int P>
+__device__ __forceinline__ int pixel_argmin(uint32_t bv, uint32_t bd)
+{
+#pragma unroll
+    for (int off = P / 2; off > 0; off >>= 1) {
+        const uint32_t ov = (uint32_t)__shfl_xor((int)bv, off, 64);
+        const uint32_t od = (uint32_t)__shfl_xor((int)bd, off, 64);
+        if (ov < bv || (ov == bv && od < bd)) {
+            bv = ov;
+            bd = od;
+        }
+    }
+    return bv == 0xFFFFFFFFu ? 0 : (int)bd;
+}
+
+// Threads per pixel of the fused sum + WTA: the power of two at or above
+// ceil(D / 4) (D <= 256).
+static int sum_wta_lanes(int D)
+{
+    const int g = (D + 3) / 4;
+    int p = 1;
+    while (p < g) p *= 2;
+    return p;
+}
+
+// Plane sum + lowest-index WTA without writing S (the caller asked only for
+// disparities): P threads per pixel (the power of two at or above
+// ceil(D / 4)), 4 cells each, the sums of sgm_wide_sum_kernel.
+template <int P, bool VEC>
+__global__ __launch_bounds__(256) void sgm_wide_sum_wta_kernel(const uint8_t *__restrict__ C, int64_t csx,
+                                                               int64_t csy, const uint8_t *__restrict__ planes,
+                                                               size_t plane_bytes, int n, int W, int H, int D,
+                                                               int16_t *__restrict__ disp, int pitch)
+{
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t pix = idx / P;
+    const int t = (int)(idx % P);
+    const bool live = pix < (int64_t)W * H; // every lane joins the shuffles
+    const int x = live ? (int)(pix % W) : 0, y = live ? (int)(pix / W) : 0;
+    const int d = 4 * t;
+    uint32_t bv = 0xFFFFFFFFu, bd = 0xFFFFFFFFu;
+    if (live && d < D) {
+        const uint8_t *cp = C + x * csx + y * csy + (int64_t)d * 4;
+        const uint8_t *ep = planes + pix * D + d;
+        uint32_t c[4], e[4] = {0, 0, 0, 0};
+        if constexpr (VEC) {
+            typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+            const u32x4 cv = __builtin_nontemporal_load((const u32x4 *)cp);
+            c[0] = cv.x, c[1] = cv.y, c[2] = cv.z, c[3] = cv.w;
+            for (int r = 0; r < n; r++) {
+                const uint32_t w = __builtin_nontemporal_load((const uint32_t *)(ep + (size_t)r * plane_bytes));
+#pragma unroll
+                for (int q = 0; q < 4; q++) e[q] += (w >> (8 * q)) & 0xFFu;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                c[q] = d + q < D ? *(const uint32_t *)(cp + 4 * q) : INV32;
+                if (d + q < D)
+                    for (int r = 0; r < n; r++) e[q] += ep[(size_t)r * plane_bytes + q];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint32_t k = wide_key(wide_s(c[q], e[q], n));
+            if (d + q < D && k < bv) {
+                bv = k;
+                bd = (uint32_t)(d + q);
+            }
+        }
+    }
+    const int best = pixel_argmin<P>(bv, bd);
+    if (live && t == 0) disp[(size_t)y * pitch + x] = (int16_t)best;
+}
+
 // Relative cost bytes of a uint32 volume (the form above): P threads per
 // pixel (the power of two at or above ceil(D / 4)), 4 cells each (VEC: one
 // 16-byte load, D % 4 == 0 and 16-byte aligned cells); the pixel's minimum
@@ -466,7 +554,7 @@ static bool wide_rel_wanted(int W, int H, int D, uint32_t p2)
 
 // The byte-plane form for uint32 volumes (P2 <= 255, planes below 2 GiB).
 static int run_wide_e(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params *p, const tdvh_dsi *S,
-                      hipStream_t s)
+                      int16_t *disp, int disp_pitch, hipStream_t s)
 {
     const int W = C->width, H = C->height, D = C->depth;
     WideEArgs a{};
@@ -529,6 +617,23 @@ static int run_wide_e(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params *p
     default: launch_wide_e_k<4>(a, s); break;
     }
     TDVH_TRY(check_launch("sgm_wide_lines_kernel", s));
+    if (!S) { // disparities only: the sum feeds the WTA directly
+        const int P = sum_wta_lanes(D);
+        const unsigned wblocks = (unsigned)(((int64_t)W * H * P + 255) / 256);
+        const bool cvec = D % 4 == 0 && C->stride_x % 4 == 0 && C->stride_y % 4 == 0 &&
+                          ((uintptr_t)C->ptr & 15) == 0;
+        prof_begin(s);
+        switch (P) {
+#define CASE(q)                                                                                    \
+    case q:                                                                                        \
+        if (cvec) sgm_wide_sum_wta_kernel<q, true><<<wblocks, 256, 0, s>>>(a.C, a.csx, a.csy, a.planes, a.plane_bytes, a.ndirs, W, H, D, disp, disp_pitch); \
+        else sgm_wide_sum_wta_kernel<q, false><<<wblocks, 256, 0, s>>>(a.C, a.csx, a.csy, a.planes, a.plane_bytes, a.ndirs, W, H, D, disp, disp_pitch); \
+        break;
+            CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32) CASE(64)
+#undef CASE
+        }
+        return check_launch("sgm_wide_sum_wta_kernel", s);
+    }
     const int64_t threads = (int64_t)W * H * ((D + 3) / 4);
     const unsigned blocks = (unsigned)((threads + 255) / 256);
     const bool vec = D % 4 == 0 && C->stride_x % 4 == 0 && C->stride_y % 4 == 0 &&
@@ -684,6 +789,52 @@ __global__ __launch_bounds__(256) void sgm_wide_fsum_kernel(const float *__restr
     }
 }
 
+// The float planes' sum + lowest-index WTA without writing S, as
+// sgm_wide_sum_wta_kernel.
+template <int P, bool VEC>
+__global__ __launch_bounds__(256) void sgm_wide_fsum_wta_kernel(const float *__restrict__ planes,
+                                                                size_t plane_elems, int n, int W, int H,
+                                                                int D, int16_t *__restrict__ disp,
+                                                                int pitch)
+{
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t pix = idx / P;
+    const int t = (int)(idx % P);
+    const bool live = pix < (int64_t)W * H; // every lane joins the shuffles
+    const int x = live ? (int)(pix % W) : 0, y = live ? (int)(pix / W) : 0;
+    const int d = 4 * t;
+    uint32_t bv = 0xFFFFFFFFu, bd = 0xFFFFFFFFu;
+    if (live && d < D) {
+        const float *lp = planes + pix * D + d;
+        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if constexpr (VEC) {
+            typedef float f32x4 __attribute__((ext_vector_type(4)));
+            f32x4 v = __builtin_nontemporal_load((const f32x4 *)lp);
+            for (int r = 1; r < n; r++)
+                v = v + __builtin_nontemporal_load((const f32x4 *)(lp + (size_t)r * plane_elems));
+            acc[0] = v.x, acc[1] = v.y, acc[2] = v.z, acc[3] = v.w;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                if (d + q >= D) break;
+                float a = lp[q];
+                for (int r = 1; r < n; r++) a = a + lp[(size_t)r * plane_elems + q];
+                acc[q] = a;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint32_t k = wide_key(acc[q]);
+            if (d + q < D && k < bv) {
+                bv = k;
+                bd = (uint32_t)(d + q);
+            }
+        }
+    }
+    const int best = pixel_argmin<P>(bv, bd);
+    if (live && t == 0) disp[(size_t)y * pitch + x] = (int16_t)best;
+}
+
 template <int K>
 static void launch_wide_l_k(const WideFArgs &a, hipStream_t s)
 {
@@ -693,7 +844,7 @@ static void launch_wide_l_k(const WideFArgs &a, hipStream_t s)
 
 // The float-plane form for float volumes (planes below 2 GiB).
 static int run_wide_f(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params *p, const tdvh_dsi *S,
-                      hipStream_t s)
+                      int16_t *disp, int disp_pitch, hipStream_t s)
 {
     const int W = C->width, H = C->height, D = C->depth;
     WideFArgs a{};
@@ -732,11 +883,26 @@ static int run_wide_f(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params *p
     default: launch_wide_l_k<4>(a, s); break;
     }
     TDVH_TRY(check_launch("sgm_wide_lines_kernel", s));
+    const size_t pe = (size_t)W * H * D;
+    if (!S) { // disparities only
+        const int P = sum_wta_lanes(D);
+        const unsigned wblocks = (unsigned)(((int64_t)W * H * P + 255) / 256);
+        prof_begin(s);
+        switch (P) {
+#define CASE(q)                                                                                    \
+    case q:                                                                                        \
+        if (D % 4 == 0) sgm_wide_fsum_wta_kernel<q, true><<<wblocks, 256, 0, s>>>((const float *)a.planes, pe, a.ndirs, W, H, D, disp, disp_pitch); \
+        else sgm_wide_fsum_wta_kernel<q, false><<<wblocks, 256, 0, s>>>((const float *)a.planes, pe, a.ndirs, W, H, D, disp, disp_pitch); \
+        break;
+            CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32) CASE(64)
+#undef CASE
+        }
+        return check_launch("sgm_wide_sum_wta_kernel", s);
+    }
     const int64_t threads = (int64_t)W * H * ((D + 3) / 4);
     const unsigned blocks = (unsigned)((threads + 255) / 256);
     const bool vec = D % 4 == 0 && S->stride_x % 4 == 0 && S->stride_y % 4 == 0 &&
                      ((uintptr_t)S->ptr & 15) == 0;
-    const size_t pe = (size_t)W * H * D;
     prof_begin(s);
     if (vec)
         sgm_wide_fsum_kernel<true><<<blocks, 256, 0, s>>>((const float *)a.planes, pe, a.ndirs,
@@ -779,11 +945,21 @@ static bool float_planes_fit(const tdvh_dsi *C)
     return !per_dir && (size_t)C->width * C->height * C->depth * 4 < ((size_t)1 << 31);
 }
 
-int run_sgm_wide(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params *p, const tdvh_dsi *S,
-                 hipStream_t s)
+// The one-launch forms (byte planes for uint32 with P2 <= 255, float planes
+// below 2 GiB), which can also feed the WTA from the sum without S.
+bool sgm_wide_one_launch(const tdvh_dsi *C, const tdvh_sgm_params *p)
 {
-    if (C->dtype == TDVH_U32 && p->p2 <= 255 && (size_t)C->width * C->height * C->depth < ((size_t)1 << 31))
-        return run_wide_e(ctx, C, p, S, s);
+    if (C->dtype == TDVH_U32)
+        return p->p2 <= 255 && (size_t)C->width * C->height * C->depth < ((size_t)1 << 31);
+    return float_planes_fit(C);
+}
+
+int run_sgm_wide(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params *p, const tdvh_dsi *S,
+                 int16_t *disp, int disp_pitch, hipStream_t s)
+{
+    TDVH_REQUIRE(S || sgm_wide_one_launch(C, p), "internal: S needed for this volume");
+    if (C->dtype == TDVH_U32 && sgm_wide_one_launch(C, p))
+        return run_wide_e(ctx, C, p, S, disp, disp_pitch, s);
     if (C->dtype == TDVH_U32) {
         WideArgs<uint32_t> a{(const uint32_t *)C->ptr, C->stride_x, C->stride_y,
                              (uint32_t *)S->ptr,       S->stride_x, S->stride_y,
@@ -791,7 +967,7 @@ int run_sgm_wide(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params *p, con
                              (uint32_t)p->p1,          (uint32_t)p->p2};
         return run_wide(a, p->dir_mask, s);
     }
-    if (float_planes_fit(C)) return run_wide_f(ctx, C, p, S, s);
+    if (float_planes_fit(C)) return run_wide_f(ctx, C, p, S, disp, disp_pitch, s);
     WideArgs<float> a{(const float *)C->ptr, C->stride_x, C->stride_y, (float *)S->ptr,
                       S->stride_x,           S->stride_y, C->width,    C->height,
                       C->depth,              p->fp1,      p->fp2};
