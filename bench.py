@@ -9,6 +9,8 @@ Prints one JSON line.
   python bench.py [--steps K] [--warmup W] [--mode fused|staged]
   python bench.py --profile-run          a few plain steps only (for rocprofv3)
   python bench.py --profile-run --shape 640x480x64   the same at another pair shape
+  python bench.py --window-only          the 1920x1080 D=256 window-cost / DP / WTA paths
+  python bench.py --pairings-only        every cost x optimiser chain on the 720p pair
   python bench.py --copy-calibration N   a plain N-byte device copy (PMC scale)
 """
 import argparse
@@ -272,6 +274,50 @@ def bench_window(tv, synth, torch, ctx, reps=5):
     return out
 
 
+def bench_pairings(tv, torch, ctx, Ld, Rd, D, reps=5):
+    """Every cost x optimiser chain of the reference's factory on the 720p
+    pair (the fused BT + SGM + WTA is the headline): the cost volume and the
+    optimiser (disparities only), with the per-kernel hipEvent times of one
+    call."""
+    W, H = Ld.shape[1], Ld.shape[0]
+    cells = W * H * D
+    fp1, fp2 = 30.0 / 255, 150.0 / 255
+    chains = {
+        "bt_wta": lambda: tv.wta(tv.cost_bt(Ld, Rd, D, ctx=ctx), ctx=ctx),
+        "bt_dp": lambda: tv.dp(tv.cost_bt(Ld, Rd, D, ctx=ctx), p_occ=20, ctx=ctx),
+        "bt_sgm_volume": lambda: tv.sgm(tv.cost_bt(Ld, Rd, D, ctx=ctx), ctx=ctx),
+        "ssd_wta": lambda: tv.wta(tv.cost_ssd(Ld, Rd, D, radius=3, ctx=ctx), ctx=ctx),
+        "ssd_dp": lambda: tv.dp(tv.cost_ssd(Ld, Rd, D, radius=3, ctx=ctx), p_occ=20,
+                                flags=tv.DP_COST_BOUNDED, ctx=ctx),
+        "ssd_sgm": lambda: tv.sgm(tv.cost_ssd(Ld, Rd, D, radius=3, ctx=ctx), ctx=ctx),
+        "ncc_wta": lambda: tv.wta(tv.cost_ncc(Ld, Rd, D, radius=4, zero_mean=True, ctx=ctx),
+                                  ctx=ctx),
+        "ncc_dp": lambda: tv.dp(tv.cost_ncc(Ld, Rd, D, radius=4, zero_mean=True, ctx=ctx),
+                                p_occ=0.1, ctx=ctx),
+        "ncc_sgm": lambda: tv.sgm(tv.cost_ncc(Ld, Rd, D, radius=4, zero_mean=True, ctx=ctx),
+                                  fp1=fp1, fp2=fp2, ctx=ctx),
+    }
+    out = {"workload": "%dx%d uint8 pair, D=%d (the headline pair); cost volume + optimiser, "
+                       "disparities only" % (W, H, D)}
+    for name, fn in chains.items():
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        ctx.profile(True)
+        ctx.profile_reset()
+        fn()
+        torch.cuda.synchronize()
+        table = ctx.profile_table()
+        ctx.profile(False)
+        out[name] = {"ms": round(ms, 3), "MDE/s": round(cells / (ms * 1e-3) / 1e6, 1),
+                     "kernels_ms": {k: round(v[0] / v[1], 4) for k, v in table.items()}}
+    return out
+
+
 def cpu_threads(L, R, D, want, threads=8, reps=2):
     """The oracle's BT + SGM-8 + WTA on `threads` host threads (each oracle
     call releases the interpreter lock): the cost volume and the plane sum +
@@ -461,6 +507,8 @@ def main():
     ap.add_argument("--profile-run", action="store_true")
     ap.add_argument("--window-only", action="store_true",
                     help="run only the 1920x1080 D=256 window-cost / DP / WTA paths (for profilers)")
+    ap.add_argument("--pairings-only", action="store_true",
+                    help="run only the cost x optimiser chains on the 720p pair (for profilers)")
     ap.add_argument("--copy-calibration", type=int, default=0)
     ap.add_argument("--shape", default="",
                     help="WxHxD of the pair for --profile-run (default: the 1280x720x128 workload)")
@@ -520,6 +568,9 @@ def main():
     if args.window_only:
         print(json.dumps({"window_1080p": bench_window(tv, synth, torch, ctx)}))
         return
+    if args.pairings_only:
+        print(json.dumps({"pairings_720p": bench_pairings(tv, torch, ctx, Ld, Rd, D)}))
+        return
     if args.profile_run:
         for _ in range(args.steps):
             step()
@@ -569,6 +620,7 @@ def main():
 
     boundary = bench_boundary(tv, torch, ctx, L, R, D)
     window = bench_window(tv, synth, torch, ctx)
+    pairings = bench_pairings(tv, torch, ctx, Ld, Rd, D)
 
     # the 384x288, D = 16 pair (seed 1): GPU call, and the oracle per stage
     plain_384 = bench_plain_384(tv, synth, torch, ctx, None if args.no_cpu_baseline else True)
@@ -633,6 +685,7 @@ def main():
         "served": served,
         "boundary": boundary,
         "window_1080p": window,
+        "pairings_720p": pairings,
         "plain_384": plain_384,
         "cpu_baseline": cpu,
         "cpu_baseline_8_threads": cpu_mt,

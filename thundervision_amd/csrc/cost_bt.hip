@@ -61,21 +61,47 @@ __global__ __launch_bounds__(256) void bt_split_kernel(const uint32_t *__restric
     }
 }
 
-// One thread per cell of a strided uint16 volume.
+// A strided uint16 volume, one grid row per band of BT_ROWS image rows: a
+// thread computes 4 consecutive cells of a pixel (G = ceil(D / 4) threads
+// per pixel) in each row of the band and stores them as one 8-byte word
+// where the cells allow it (VEC: D, both strides multiples of 4 cells and
+// the base 8-byte aligned).
+constexpr int BT_ROWS = 8;
+template <bool VEC, bool POW2>
 __global__ __launch_bounds__(256) void bt_strided_kernel(const uint32_t *__restrict__ TL,
                                                          const uint32_t *__restrict__ TR, int W,
-                                                         int H, int D, int64_t sx, int64_t sy,
+                                                         int H, int D, int G, int gshift,
+                                                         int64_t sx, int64_t sy,
                                                          uint16_t *__restrict__ out)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t n = (int64_t)W * H * D;
-    if (i >= n) return;
-    const int d = (int)(i % D);
-    const int64_t px = i / D;
-    const int x = (int)(px % W), y = (int)(px / W);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= W * G) return;
+    const int x = POW2 ? i >> gshift : i / G, d0 = 4 * (i - x * G);
+    const int y1 = min((int)blockIdx.y * BT_ROWS + BT_ROWS, H);
+    for (int y = blockIdx.y * BT_ROWS; y < y1; y++) {
     const size_t p = (size_t)y * W + x;
-    const uint32_t c = d <= x ? bt_cost(TL[p], TR[p - d]) : INV16;
-    out[x * sx + y * sy + d] = (uint16_t)c;
+    const uint32_t lt = TL[p];
+    const int l2 = lt & 0x3FF, lmin = (lt >> 10) & 0x3FF, lmax = (lt >> 20) & 0x3FF;
+    uint32_t c[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int d = d0 + q;
+        // the right pixel x - d >= 0 always has terms (no BT_NONE here)
+        const uint32_t rt = TR[p - (d <= x ? d : 0)];
+        const int r2 = rt & 0x3FF, rmin = (rt >> 10) & 0x3FF, rmax = (rt >> 20) & 0x3FF;
+        const int a = max(0, max(l2 - rmax, rmin - l2));
+        const int b = max(0, max(r2 - lmax, lmin - r2));
+        c[q] = d < D && d <= x ? (uint32_t)min(a, b) : INV16;
+    }
+    uint16_t *o = out + x * sx + (int64_t)y * sy + d0;
+    if constexpr (VEC) {
+        *(uint2 *)o = make_uint2(c[0] | (c[1] << 16), c[2] | (c[3] << 16));
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            if (d0 + q < D) o[q] = (uint16_t)c[q];
+    }
+    }
 }
 
 int launch_bt_split(const uint32_t *TL, const uint32_t *TR, int W, int H, int D, uint32_t *out,
@@ -97,17 +123,18 @@ int launch_bt_split(const uint32_t *TL, const uint32_t *TR, int W, int H, int D,
 
 // ---- DSI layout conversion (uint16 strided <-> SPLIT16) --------------------
 
+// One grid row per image row, a thread per SPLIT16 word.
 template <bool TO_SPLIT>
 __global__ __launch_bounds__(256) void convert_kernel(uint16_t *strided, uint32_t *split, int W,
                                                       int H, int D, int M, int64_t sx,
                                                       int64_t sy)
 {
     const int WPP = 16 * M;
-    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= (int64_t)W * H * WPP) return;
-    const int64_t p = w / WPP;
-    const int jr = (int)(w - p * WPP);
-    const int x = (int)(p % W), y = (int)(p / W);
+    const int y = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= W * WPP) return;
+    const int x = i / WPP, jr = i - x * WPP;
+    const int64_t w = (int64_t)y * W * WPP + i;
     const int dlo = jr, dhi = 16 * M + jr;
     uint16_t *base = strided + x * sx + y * sy;
     if (TO_SPLIT) {
@@ -125,13 +152,12 @@ int launch_convert(uint16_t *strided, int64_t sx, int64_t sy, uint32_t *split, i
                    int D, bool to_split, hipStream_t s)
 {
     const int M = split_m(D);
-    const int64_t n = (int64_t)W * H * 16 * M;
-    const int blocks = (int)((n + 255) / 256);
+    const dim3 grid((unsigned)((W * 16 * M + 255) / 256), (unsigned)H);
     prof_begin(s);
     if (to_split)
-        convert_kernel<true><<<blocks, 256, 0, s>>>(strided, split, W, H, D, M, sx, sy);
+        convert_kernel<true><<<grid, 256, 0, s>>>(strided, split, W, H, D, M, sx, sy);
     else
-        convert_kernel<false><<<blocks, 256, 0, s>>>(strided, split, W, H, D, M, sx, sy);
+        convert_kernel<false><<<grid, 256, 0, s>>>(strided, split, W, H, D, M, sx, sy);
     return check_launch("convert_kernel", s);
 }
 
@@ -161,10 +187,20 @@ extern "C" int tdvh_cost_bt(tdvh_ctx *ctx, const uint8_t *left, const uint8_t *r
         return launch_bt_split(TL, TR, width, height, out->depth, (uint32_t *)out->ptr, s);
     }
     TDVH_REQUIRE(out->layout == TDVH_LAYOUT_STRIDED, "unknown layout %d", out->layout);
-    const int64_t n = (int64_t)width * height * out->depth;
+    const int D = out->depth, G = (D + 3) / 4;
+    TDVH_REQUIRE((int64_t)width * G < (1ll << 31), "image width %d too large", width);
+    const dim3 grid((unsigned)((width * G + 255) / 256), (unsigned)((height + BT_ROWS - 1) / BT_ROWS));
+    const bool vec = D % 4 == 0 && out->stride_x % 4 == 0 && out->stride_y % 4 == 0 &&
+                     ((uintptr_t)out->ptr & 7) == 0;
+    const bool pow2 = (G & (G - 1)) == 0;
+    const int gshift = __builtin_ctz((unsigned)G);
+    uint16_t *o = (uint16_t *)out->ptr;
+    const int64_t sx = out->stride_x, sy = out->stride_y;
     prof_begin(s);
-    bt_strided_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(
-        TL, TR, width, height, out->depth, out->stride_x, out->stride_y, (uint16_t *)out->ptr);
+    if (vec && pow2) bt_strided_kernel<true, true><<<grid, 256, 0, s>>>(TL, TR, width, height, D, G, gshift, sx, sy, o);
+    else if (vec) bt_strided_kernel<true, false><<<grid, 256, 0, s>>>(TL, TR, width, height, D, G, gshift, sx, sy, o);
+    else if (pow2) bt_strided_kernel<false, true><<<grid, 256, 0, s>>>(TL, TR, width, height, D, G, gshift, sx, sy, o);
+    else bt_strided_kernel<false, false><<<grid, 256, 0, s>>>(TL, TR, width, height, D, G, gshift, sx, sy, o);
     return check_launch("bt_strided_kernel", s);
 }
 
