@@ -469,7 +469,7 @@ extern "C" int tdvh_batch_create(tdvh_ctx *ctx, const tdvh_pair *pairs, int n,
         off_planes[i] = total;
         total = align(total + 8 * W * H * 32 * m);
         off_rel[i] = 0;
-        if (sgm_rel_auto(p.width, p.height, m, params->p2)) {
+        if (sgm_rel_auto(p.width, p.height, m, params->p2, false)) {
             off_rel[i] = total;
             total = align(total + W * H * 32 * m);
         }

@@ -31,10 +31,11 @@ int run_sgm_wide(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params *p, con
 
 int popcount8(int mask);
 
-// Whether the fused BT + SGM scans of a W x H pair at m words per lane and
-// penalty P2 read relative cost bytes (sgm_rel_pixel) rather than
-// recomputing BT (the choice of the per-pair path, sgm.hip).
-bool sgm_rel_auto(int W, int H, int m, int p2);
+// Whether the SGM scans of a W x H pair at m words per lane and penalty P2
+// read relative cost bytes (sgm_rel_pixel) rather than recomputing BT
+// (fused) or reading the uint16 volume (volume true): the choice of the
+// per-pair path, sgm.hip.
+bool sgm_rel_auto(int W, int H, int m, int p2, bool volume);
 
 // Appends the line-group table of one W x H pair (entries plane slot << 28 |
 // direction << 24 | first line; lpw lines per group) to `out`, ordered

@@ -89,19 +89,24 @@ static void launch_lines_buf(const SgmArgs &a, const uint32_t *order, int groups
 // faster). At 118 MB of cost bytes (1280x720, D = 128) the result varies
 // from device to device (-2 % to +7 %), and beyond (1280x720 D = 256,
 // 1920x1080 D >= 128) the byte scans are 10-20 % slower.
-static bool rel_wanted(int W, int H, int m)
+// Fused (BT terms): the bytes replace the scans' BT work with a read per
+// cell and direction, which pays for D <= 64 and up to 96 MiB of bytes.
+// A given uint16 volume: the bytes halve the scans' cost reads (2 bytes per
+// cell and direction to 1), measured faster at every shape (1280x720 D =
+// 128: 0.715 vs 0.729 ms, 1920x1080 D = 256: 3.01 vs 3.50 ms per pair).
+static bool rel_wanted(int W, int H, int m, bool volume)
 {
     static const int mode = [] {
         const char *e = getenv("TDVH_SGM_REL");
         return e ? atoi(e) : -1;
     }();
     if (mode >= 0) return mode != 0;
-    return m <= 2 || (size_t)W * H * 32 * m <= ((size_t)96 << 20);
+    return volume || m <= 2 || (size_t)W * H * 32 * m <= ((size_t)96 << 20);
 }
 
-bool sgm_rel_auto(int W, int H, int m, int p2)
+bool sgm_rel_auto(int W, int H, int m, int p2, bool volume)
 {
-    return p2 <= 63 && (m == 1 || m % 2 == 0) && rel_wanted(W, H, m);
+    return p2 <= 63 && (m == 1 || m % 2 == 0) && rel_wanted(W, H, m, volume);
 }
 
 template <int M>
@@ -520,7 +525,7 @@ static int sgm_std(tdvh_ctx *ctx, const uint32_t *Csplit, const uint8_t *left,
     }
     // relative cost bytes (byte planes, 2 P2 below 128, an even word count,
     // buffer-addressed scans)
-    if (eout && sgm_rel_auto(W, H, m, params->p2) && lines_buf_enabled()) {
+    if (eout && sgm_rel_auto(W, H, m, params->p2, Csplit != nullptr) && lines_buf_enabled()) {
         SgmArgs r = a;
         r.CR = reinterpret_cast<uint8_t *>(1); // sized below; checked by buf_lines_fit
         if (buf_lines_fit(r)) {
