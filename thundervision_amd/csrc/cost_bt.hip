@@ -148,9 +148,67 @@ __global__ __launch_bounds__(256) void convert_kernel(uint16_t *strided, uint32_
     }
 }
 
+// Four consecutive SPLIT16 words per thread (d = j..j+3 in the low halves,
+// 16M + j.. in the high): an 8-byte access per half and a 16-byte one to
+// the words, where the volume allows (both strides multiples of 4 cells,
+// 8-byte aligned strided base, 16-byte aligned words); cells at or above D
+// by element.
+template <bool TO_SPLIT>
+__global__ __launch_bounds__(256) void convert4_kernel(uint16_t *strided, uint32_t *split, int W,
+                                                       int D, int M, int64_t sx, int64_t sy)
+{
+    const int WPP = 16 * M, Q = 4 * M; // words and quads per pixel
+    const int y = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= W * Q) return;
+    const int x = i / Q, j0 = 4 * (i - x * Q);
+    uint4 *wp = (uint4 *)(split + ((int64_t)y * W + x) * WPP + j0);
+    uint16_t *base = strided + x * sx + y * sy;
+    const int dlo = j0, dhi = 16 * M + j0;
+    auto get = [&](int d0, uint32_t(&v)[4]) {
+        if (d0 + 3 < D) {
+            const uint2 u = *(const uint2 *)(base + d0);
+            v[0] = u.x & 0xFFFFu, v[1] = u.x >> 16, v[2] = u.y & 0xFFFFu, v[3] = u.y >> 16;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; q++) v[q] = d0 + q < D ? base[d0 + q] : INV16;
+        }
+    };
+    auto put = [&](int d0, const uint32_t(&v)[4]) {
+        if (d0 + 3 < D) {
+            *(uint2 *)(base + d0) = make_uint2(v[0] | (v[1] << 16), v[2] | (v[3] << 16));
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (d0 + q < D) base[d0 + q] = (uint16_t)v[q];
+        }
+    };
+    if constexpr (TO_SPLIT) {
+        uint32_t lo[4], hi[4];
+        get(dlo, lo);
+        get(dhi, hi);
+        *wp = make_uint4(lo[0] | (hi[0] << 16), lo[1] | (hi[1] << 16), lo[2] | (hi[2] << 16),
+                         lo[3] | (hi[3] << 16));
+    } else {
+        const uint4 w = *wp;
+        const uint32_t lo[4] = {w.x & 0xFFFFu, w.y & 0xFFFFu, w.z & 0xFFFFu, w.w & 0xFFFFu};
+        const uint32_t hi[4] = {w.x >> 16, w.y >> 16, w.z >> 16, w.w >> 16};
+        put(dlo, lo);
+        put(dhi, hi);
+    }
+}
+
 int launch_convert(uint16_t *strided, int64_t sx, int64_t sy, uint32_t *split, int W, int H,
                    int D, bool to_split, hipStream_t s)
 {
+    if (sx % 4 == 0 && sy % 4 == 0 && ((uintptr_t)strided & 7) == 0 && ((uintptr_t)split & 15) == 0) {
+        const int M = split_m(D);
+        const dim3 grid((unsigned)((W * 4 * M + 255) / 256), (unsigned)H);
+        prof_begin(s);
+        if (to_split) convert4_kernel<true><<<grid, 256, 0, s>>>(strided, split, W, D, M, sx, sy);
+        else convert4_kernel<false><<<grid, 256, 0, s>>>(strided, split, W, D, M, sx, sy);
+        return check_launch("convert_kernel", s);
+    }
     const int M = split_m(D);
     const dim3 grid((unsigned)((W * 16 * M + 255) / 256), (unsigned)H);
     prof_begin(s);
