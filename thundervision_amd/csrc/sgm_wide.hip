@@ -493,6 +493,50 @@ __global__ __launch_bounds__(256) void sgm_wide_sum_wta_kernel(const uint8_t *__
     if (live && t == 0) disp[(size_t)y * pitch + x] = (int16_t)best;
 }
 
+// The same with 16 cells per thread (D % 16 == 0, 16-byte aligned cells):
+// 16-byte loads of C and of every plane.
+template <int P>
+__global__ __launch_bounds__(256) void sgm_wide_sum_wta16_kernel(const uint8_t *__restrict__ C, int64_t csx,
+                                                                 int64_t csy, const uint8_t *__restrict__ planes,
+                                                                 size_t plane_bytes, int n, int W, int H, int D,
+                                                                 int16_t *__restrict__ disp, int pitch)
+{
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t pix = idx / P;
+    const int t = (int)(idx % P);
+    const bool live = pix < (int64_t)W * H; // every lane joins the shuffles
+    const int x = live ? (int)(pix % W) : 0, y = live ? (int)(pix / W) : 0;
+    const int d = 16 * t;
+    uint32_t bv = 0xFFFFFFFFu, bd = 0xFFFFFFFFu;
+    if (live && d < D) {
+        const uint8_t *cp = C + x * csx + y * csy + (int64_t)d * 4;
+        const uint8_t *ep = planes + pix * D + d;
+        uint32_t e[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) e[q] = 0;
+        for (int r = 0; r < n; r++) {
+            const u32x4 w = __builtin_nontemporal_load((const u32x4 *)(ep + (size_t)r * plane_bytes));
+#pragma unroll
+            for (int q = 0; q < 16; q++) e[q] += (w[q / 4] >> (8 * (q % 4))) & 0xFFu;
+        }
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+            const u32x4 c = __builtin_nontemporal_load((const u32x4 *)(cp + 16 * g));
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t k = wide_key(wide_s(c[q], e[4 * g + q], n));
+                if (k < bv) {
+                    bv = k;
+                    bd = (uint32_t)(d + 4 * g + q);
+                }
+            }
+        }
+    }
+    const int best = pixel_argmin<P>(bv, bd);
+    if (live && t == 0) disp[(size_t)y * pitch + x] = (int16_t)best;
+}
+
 // Relative cost bytes of a uint32 volume (the form above): P threads per
 // pixel (the power of two at or above ceil(D / 4)), 4 cells each (VEC: one
 // 16-byte load, D % 4 == 0 and 16-byte aligned cells); the pixel's minimum
@@ -618,10 +662,22 @@ static int run_wide_e(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params *p
     }
     TDVH_TRY(check_launch("sgm_wide_lines_kernel", s));
     if (!S) { // disparities only: the sum feeds the WTA directly
-        const int P = sum_wta_lanes(D);
-        const unsigned wblocks = (unsigned)(((int64_t)W * H * P + 255) / 256);
         const bool cvec = D % 4 == 0 && C->stride_x % 4 == 0 && C->stride_y % 4 == 0 &&
                           ((uintptr_t)C->ptr & 15) == 0;
+        if (cvec && D % 16 == 0) {
+            const int P16 = sum_wta_lanes(D / 4); // lanes of 16 cells
+            const unsigned b16 = (unsigned)(((int64_t)W * H * P16 + 255) / 256);
+            prof_begin(s);
+            switch (P16) {
+#define CASE(q)                                                                                    \
+    case q: sgm_wide_sum_wta16_kernel<q><<<b16, 256, 0, s>>>(a.C, a.csx, a.csy, a.planes, a.plane_bytes, a.ndirs, W, H, D, disp, disp_pitch); break;
+                CASE(1) CASE(2) CASE(4) CASE(8) CASE(16)
+#undef CASE
+            }
+            return check_launch("sgm_wide_sum_wta_kernel", s);
+        }
+        const int P = sum_wta_lanes(D);
+        const unsigned wblocks = (unsigned)(((int64_t)W * H * P + 255) / 256);
         prof_begin(s);
         switch (P) {
 #define CASE(q)                                                                                    \
