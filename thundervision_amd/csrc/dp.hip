@@ -53,6 +53,8 @@
 // then the whole wave walks the columns in lockstep with a wave-uniform
 // disparity, reading the decision bits of lane z / K by readlane.
 #include <atomic>
+#include <cstdlib>
+#include <string>
 #include <type_traits>
 
 #include "kernels.hpp"
@@ -215,7 +217,8 @@ __global__ __launch_bounds__(64) void dp_kernel(const TIn *__restrict__ C, int W
                                                 float Pf, int flags, uint8_t *__restrict__ path,
                                                 int16_t *__restrict__ disp, int pitch,
                                                 const uint8_t *__restrict__ only,
-                                                uint8_t *__restrict__ flags_out)
+                                                uint8_t *__restrict__ flags_out, size_t rstride,
+                                                uint32_t bstride)
 {
     static_assert(LD == LD_ELEM || (K * sizeof(TIn)) % 4 == 0, "dword loads need whole dwords per lane");
     constexpr int CPW = dp_cpw(K), WORD_BITS = 2 * K * CPW; // decision bits per word
@@ -232,8 +235,10 @@ __global__ __launch_bounds__(64) void dp_kernel(const TIn *__restrict__ C, int W
         const int m = min(n, W - x);
         return buf_rsrc(crow + x * sx, m > 0 ? (uint32_t)(m - 1) * sxb + cellb : 0u);
     };
-    const uint32_t prow_bytes = (uint32_t)dp_row_bytes(W, K);
-    const buf_t prs = buf_rsrc(path + (size_t)y * prow_bytes, prow_bytes);
+    // decision word block b (columns [b CPW, (b + 1) CPW)) of this row at
+    // byte b * bstride of the resource (words past the row read 0)
+    const uint32_t nblk = (uint32_t)((W + CPW - 1) / CPW);
+    const buf_t prs = buf_rsrc(path + (size_t)y * rstride, (nblk - 1) * bstride + 256u);
     const uint32_t lane_off = (uint32_t)(l * K * (int)sizeof(TIn));
 
     auto load = [&](cval_t(&c)[K], buf_t rs, uint32_t off) {
@@ -373,7 +378,7 @@ __global__ __launch_bounds__(64) void dp_kernel(const TIn *__restrict__ C, int W
         if (slot == CPW - 1 || (tail && x == W - 1)) {
             // a partial last word moves up to the full word's positions
             __builtin_amdgcn_raw_buffer_store_b32(cw << ((CPW - 1 - slot) * 2 * K), prs,
-                                                  st_off + (uint32_t)((1 + u) / CPW) * 256u, 0, 0);
+                                                  st_off + (uint32_t)((1 + u) / CPW) * bstride, 0, 0);
             cw = 0;
         }
         // refill the slot, PF columns ahead, once its values are consumed
@@ -392,7 +397,7 @@ __global__ __launch_bounds__(64) void dp_kernel(const TIn *__restrict__ C, int W
     // the bounded contract exactly, the others keep a running maximum.
     static_assert(PF % 4 == 0, "whole decision words and renormalisation periods per block");
     int x0 = 1;
-    constexpr uint32_t st_blk = (uint32_t)(PF / CPW) * 256u;
+    const uint32_t st_blk = (uint32_t)(PF / CPW) * bstride;
     for (; x0 + PF <= W && (B32 && x0 < 64 * K); x0 += PF, st_off += st_blk) {
         next = cols(x0 + PF, PF);
 #pragma unroll
@@ -407,7 +412,7 @@ __global__ __launch_bounds__(64) void dp_kernel(const TIn *__restrict__ C, int W
     if (x0 == W) { // the last block ended the row inside a word
         if ((W - 1) % CPW != CPW - 1)
             __builtin_amdgcn_raw_buffer_store_b32(cw << ((CPW - 1 - (W - 1) % CPW) * 2 * K), prs,
-                                                  (uint32_t)((W - 1) / CPW) * 256u + l * 4, 0, 0);
+                                                  (uint32_t)((W - 1) / CPW) * bstride + l * 4, 0, 0);
     }
 #pragma unroll
     for (int u = 0; u < PF; u++)
@@ -463,7 +468,7 @@ __global__ __launch_bounds__(64) void dp_kernel(const TIn *__restrict__ C, int W
     auto fetch = [&](int wb, uint32_t(&wv)[WPB]) {
 #pragma unroll
         for (int i = 0; i < WPB; i++) // words past the row read 0
-            wv[i] = __builtin_amdgcn_raw_buffer_load_b32(prs, (uint32_t)(wb + i) * 256u + l * 4, 0, 0);
+            wv[i] = __builtin_amdgcn_raw_buffer_load_b32(prs, (uint32_t)(wb + i) * bstride + l * 4, 0, 0);
     };
     auto walk = [&](int wb, uint32_t(&wv)[WPB], bool partial) {
 #pragma unroll
@@ -513,7 +518,9 @@ struct DpLaunch {
     int pitch;
     const uint8_t *only;
     uint8_t *flags_out;
-    int ld; // LD_ELEM, LD_DW or LD_FULL
+    int ld;          // LD_ELEM, LD_DW or LD_FULL
+    size_t rstride;  // bytes between the decision words of rows y and y + 1
+    uint32_t bstride; // bytes between a row's word blocks b and b + 1
 };
 
 // Prefetch depth (columns of costs in flight per row): 16 for K <= 4, 8
@@ -524,7 +531,7 @@ static void launch_dp_k(const DpLaunch &a, hipStream_t s)
     constexpr int PF = K > 4 ? 8 : 16;
     dp_kernel<TIn, K, PF, LD, MODE><<<a.H, 64, 0, s>>>(
         (const TIn *)a.C, a.W, a.H, a.D, a.sx, a.sy, a.P, a.Pf, a.flags, a.path, a.disp,
-        a.pitch, a.only, a.flags_out);
+        a.pitch, a.only, a.flags_out, a.rstride, a.bstride);
 }
 
 template <typename TIn, int K, int MODE>
@@ -555,6 +562,29 @@ static void launch_dp(const DpLaunch &a, hipStream_t s)
     case 4: launch_dp_ld<TIn, 4, MODE>(a, s); break;
     case 6: launch_dp_ld<TIn, 6, MODE>(a, s); break;
     default: launch_dp_ld<TIn, 8, MODE>(a, s); break;
+    }
+}
+
+// Where the decision words go. Rows interleaved by word block (block b of
+// row y at (b H + y) 256 bytes): every row's wave is at about the same
+// column, so at any time the rows' stores, and later the backtracks' loads,
+// fall in one contiguous band, as the cost reads of a strided volume do.
+// Row-major (row y's blocks contiguous from y * row bytes) where the
+// interleaved offsets would not fit 32 bits, or with TDVH_DP_WORDS=rows.
+static void dp_word_layout(int W, int H, int K, size_t *rstride, uint32_t *bstride)
+{
+    static const bool rows = [] {
+        const char *e = getenv("TDVH_DP_WORDS");
+        return e && std::string(e) == "rows";
+    }();
+    const size_t nblk = (size_t)(W + dp_cpw(K) - 1) / dp_cpw(K);
+    // the backtrack's fetch reads up to 64 / CPW blocks past the row's last
+    if (!rows && (nblk + 32) * (size_t)H * 256 < (1ull << 32)) {
+        *rstride = 256;
+        *bstride = (uint32_t)H * 256u;
+    } else {
+        *rstride = dp_row_bytes(W, K);
+        *bstride = 256;
     }
 }
 
@@ -789,14 +819,15 @@ __device__ __forceinline__ void wave_best(const uint32_t (&A)[K], int d0, uint32
 // current one is walked; the wave walks with a wave-uniform z), as
 // dp_kernel's.
 template <int K>
-__device__ __forceinline__ void dp_ssd_backtrack(buf_t prs, int W, int D, bool refbt, uint32_t z,
-                                 int16_t *__restrict__ out, int l)
+__device__ __forceinline__ void dp_ssd_backtrack(buf_t prs, uint32_t bstride, int W, int D,
+                                                 bool refbt, uint32_t z,
+                                                 int16_t *__restrict__ out, int l)
 {
     constexpr int CPW = dp_cpw(K), WORD_BITS = 2 * K * CPW, WPB = 64 / CPW;
     auto fetch = [&](int wb, uint32_t(&wv)[WPB]) {
 #pragma unroll
         for (int i = 0; i < WPB; i++)
-            wv[i] = __builtin_amdgcn_raw_buffer_load_b32(prs, (uint32_t)(wb + i) * 256u + l * 4, 0, 0);
+            wv[i] = __builtin_amdgcn_raw_buffer_load_b32(prs, (uint32_t)(wb + i) * bstride + l * 4, 0, 0);
     };
     auto walk = [&](int wb, uint32_t(&wv)[WPB], bool partial) {
 #pragma unroll
@@ -838,7 +869,8 @@ __device__ __forceinline__ void dp_ssd_backtrack(buf_t prs, int W, int D, bool r
 // staged columns, storing the decision words; returns the row's start.
 template <int K, int RAD>
 __device__ __forceinline__ uint32_t dp_ssd_scan(const uint2 *lc, const uint2 *rc, int W, int D,
-                                                uint32_t P, int flags, buf_t prs, int l)
+                                                uint32_t P, int flags, buf_t prs,
+                                                uint32_t bstride, int l)
 {
     static_assert(dp_cpw(K) == 4, "four columns per decision word");
     constexpr int NR = 2 * RAD + 1, UNR = dp_ssd_unroll(RAD);
@@ -855,7 +887,7 @@ __device__ __forceinline__ uint32_t dp_ssd_scan(const uint2 *lc, const uint2 *rc
         fs.costs(x, ri, slot, edge, c);
         dl.step(c, P, slot);
         if (slot == 3) { // compile-time per unrolled step: no branch in the loop body
-            __builtin_amdgcn_raw_buffer_store_b32(dl.cw, prs, (uint32_t)(x >> 2) * 256u + l * 4, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(dl.cw, prs, (uint32_t)(x >> 2) * bstride + l * 4, 0, 0);
             dl.cw = 0;
         }
     };
@@ -877,7 +909,7 @@ __device__ __forceinline__ uint32_t dp_ssd_scan(const uint2 *lc, const uint2 *rc
     // included) moves up to the full word's bits
     if (((W - 1) & 3) != 3)
         __builtin_amdgcn_raw_buffer_store_b32(dl.cw << ((3 - ((W - 1) & 3)) * 2 * K), prs,
-                                              (uint32_t)((W - 1) >> 2) * 256u + l * 4, 0, 0);
+                                              (uint32_t)((W - 1) >> 2) * bstride + l * 4, 0, 0);
     // start: lowest-index argmin of the last column (or d = 0)
     if (flags & TDVH_DP_START_ZERO) return 0;
     uint32_t bv;
@@ -891,7 +923,8 @@ __global__ __launch_bounds__(64) void dp_ssd_kernel(const uint8_t *__restrict__ 
                                                     const uint8_t *__restrict__ Rimg, int W,
                                                     int H, int pitch, int D, uint32_t P,
                                                     int flags, uint8_t *__restrict__ path,
-                                                    int16_t *__restrict__ disp, int dpitch)
+                                                    int16_t *__restrict__ disp, int dpitch,
+                                                    size_t rstride, uint32_t bstride)
 {
     static_assert(K >= 1 && K <= 4 && RAD >= 1 && RAD <= 3, "K <= 4, radius 1..3");
     extern __shared__ uint2 cols_lds[];
@@ -900,11 +933,13 @@ __global__ __launch_bounds__(64) void dp_ssd_kernel(const uint8_t *__restrict__ 
     uint2 *const lc = cols_lds, *const rc = cols_lds + WP;
     ssd_stage_rows<RAD>(Limg, Rimg, W, H, pitch, y, lc, rc, l, 64);
     __syncthreads();
-    const uint32_t prow_bytes = (uint32_t)dp_row_bytes(W, K);
-    const buf_t prs = buf_rsrc(path + (size_t)y * prow_bytes, prow_bytes);
-    const uint32_t start = dp_ssd_scan<K, RAD>(lc, rc, W, D, P, flags, prs, l);
+    // decision word block b of this row at byte b * bstride
+    const uint32_t nblk = (uint32_t)((W + 3) / 4);
+    const buf_t prs = buf_rsrc(path + (size_t)y * rstride, (nblk - 1) * bstride + 256u);
+    const uint32_t start = dp_ssd_scan<K, RAD>(lc, rc, W, D, P, flags, prs, bstride, l);
     const uint32_t z = (uint32_t)__builtin_amdgcn_readfirstlane((int)start);
-    dp_ssd_backtrack<K>(prs, W, D, flags & TDVH_DP_REF_BACKTRACK, z, disp + (size_t)y * dpitch, l);
+    dp_ssd_backtrack<K>(prs, bstride, W, D, flags & TDVH_DP_REF_BACKTRACK, z,
+                        disp + (size_t)y * dpitch, l);
 }
 
 // LDS bytes per workgroup of a device, queried once (64 KiB, what every
@@ -929,10 +964,13 @@ static void launch_dp_ssd_k(const uint8_t *L, const uint8_t *R, int W, int H, in
                             int dpitch, hipStream_t s)
 {
     const size_t lds = (size_t)(W + radius + 1) * 16;
+    size_t rs;
+    uint32_t bs;
+    dp_word_layout(W, H, K, &rs, &bs);
     switch (radius) {
-    case 1: dp_ssd_kernel<K, 1><<<H, 64, lds, s>>>(L, R, W, H, pitch, D, P, flags, path, disp, dpitch); break;
-    case 2: dp_ssd_kernel<K, 2><<<H, 64, lds, s>>>(L, R, W, H, pitch, D, P, flags, path, disp, dpitch); break;
-    default: dp_ssd_kernel<K, 3><<<H, 64, lds, s>>>(L, R, W, H, pitch, D, P, flags, path, disp, dpitch); break;
+    case 1: dp_ssd_kernel<K, 1><<<H, 64, lds, s>>>(L, R, W, H, pitch, D, P, flags, path, disp, dpitch, rs, bs); break;
+    case 2: dp_ssd_kernel<K, 2><<<H, 64, lds, s>>>(L, R, W, H, pitch, D, P, flags, path, disp, dpitch, rs, bs); break;
+    default: dp_ssd_kernel<K, 3><<<H, 64, lds, s>>>(L, R, W, H, pitch, D, P, flags, path, disp, dpitch, rs, bs); break;
     }
 }
 
@@ -990,7 +1028,8 @@ static int dp_run(tdvh_ctx *ctx, const tdvh_dsi *C, uint32_t p_occ, float pf, in
                                  C->stride_y % 2 == 0 && K % 2 == 0);
     DpLaunch a{C->ptr, W, H, D, C->stride_x, C->stride_y,
                p_occ, pf, flags, (uint8_t *)pb, disp, disp_pitch, nullptr, nullptr,
-               !dw ? LD_ELEM : (D == 64 * K ? LD_FULL : LD_DW)};
+               !dw ? LD_ELEM : (D == 64 * K ? LD_FULL : LD_DW), 0, 0};
+    dp_word_layout(W, H, K, &a.rstride, &a.bstride);
     if (C->dtype == TDVH_F32) {
         prof_begin(s);
         launch_dp<float, DP_F32>(a, s);
