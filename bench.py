@@ -130,7 +130,7 @@ def bench_batched(tv, synth, torch, ctx, reps, specs=None, workload=MIX_WORKLOAD
             "plain_per_pair_value": round(cells / (ms_plain * 1e-3) / 1e6, 1),
             "speedup_vs_plain": round(ms_plain / ms_graph, 3),
             "tuned": {k: tuned[k] for k in ("fusion_depth", "lines_per_wave", "waves_per_group",
-                                              "prefetch", "cached")},
+                                              "prefetch", "sum_lag", "cached")},
             "tune_trials_ms": [dict(t, ms=round(t["ms"], 3)) for t in tuned["trials"]],
             "bit_exact_vs_plain": exact}
 
