@@ -131,6 +131,26 @@ def test_join_cells_runs_large_frames_one_at_a_time_in_arrival_order():
     assert all(a[1] <= b[0] + 1e-12 for a, b in zip(spans, spans[1:]))
 
 
+def test_smallest_first_with_an_age_bound():
+    # one slot; while a large frame runs, a large, a medium and a small frame
+    # queue up in that order: smallest first starts them small, medium,
+    # large; with an age bound they have all passed, arrival order decides
+    big, mid, small = (1920, 1080, 256), (1280, 720, 128), (320, 240, 32)
+    def order(**kw):
+        frames = [serve.Frame(0.0, 0, 0, *big), serve.Frame(0.001, 1, 0, *big),
+                  serve.Frame(0.0015, 2, 0, *mid), serve.Frame(0.002, 3, 0, *small)]
+        ex = FakeExecutor(rate=2e10)  # the first frame takes ~26 ms
+        done = serve.Scheduler(ex, slots=1, budget_bytes=64 << 30, **kw).run(frames)
+        return [f.stream for f in sorted(done, key=lambda f: f.t_start)]
+    assert order() == [0, 1, 2, 3]
+    assert order(smallest_first=True) == [0, 3, 2, 1]
+    assert order(smallest_first=True, max_wait_s=0.05) == [0, 3, 2, 1]
+    assert order(smallest_first=True, max_wait_s=0.01) == [0, 1, 2, 3]
+    assert order(largest_first=True) == [0, 1, 2, 3]
+    with pytest.raises(ValueError):
+        serve.Scheduler(FakeExecutor(), 1, 1 << 30, largest_first=True, smallest_first=True)
+
+
 def test_gc_is_paused_while_serving_and_restored():
     import gc
     ex = FakeExecutor()

@@ -57,7 +57,9 @@ class Scheduler:
     executor.now() is the schedule clock.
 
     Admission is in arrival order (largest_first: the largest queued frame
-    first, then arrival order), needs a free slot and memory within
+    first, then arrival order; smallest_first: the smallest first, except
+    that frames queued for max_wait_s seconds or longer go before all others,
+    in arrival order), needs a free slot and memory within
     budget_bytes, and, with cell_budget, keeps the cells (W*H*D) of the
     frames in flight within that budget, so that large frames do not share
     the device with each other (a frame always starts on an idle device).
@@ -70,7 +72,9 @@ class Scheduler:
     """
 
     def __init__(self, executor, slots, budget_bytes, cell_budget=None, backfill_s=None,
-                 largest_first=False, join_cells=None):
+                 largest_first=False, join_cells=None, smallest_first=False, max_wait_s=None):
+        if largest_first and smallest_first:
+            raise ValueError("largest_first and smallest_first exclude each other")
         self.ex = executor
         self.nslots = slots
         self.budget = budget_bytes
@@ -78,6 +82,8 @@ class Scheduler:
         self.backfill_s = backfill_s
         self.largest_first = largest_first
         self.join_cells = join_cells
+        self.smallest_first = smallest_first
+        self.max_wait_s = max_wait_s
         self.in_use = 0
         self.in_flight_cells = 0
         self.peak_in_use = 0
@@ -125,6 +131,11 @@ class Scheduler:
             # blocked head frame)
             if self.largest_first and len(queue) > 1:
                 queue = collections.deque(sorted(queue, key=lambda f: (-f.cells, f.t_arrival)))
+            elif self.smallest_first and len(queue) > 1:
+                aged = self.max_wait_s if self.max_wait_s is not None else float("inf")
+                queue = collections.deque(sorted(
+                    queue, key=lambda f: (0, f.t_arrival, 0) if now - f.t_arrival >= aged
+                    else (1, f.cells, f.t_arrival)))
             while queue and free:
                 pick = None
                 for j, f in enumerate(queue):
