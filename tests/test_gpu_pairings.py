@@ -94,7 +94,8 @@ names = set()
 rng = np.random.default_rng(5)
 for (W, H, D, p1, p2, mask) in [(60, 33, 16, 7, 32, 0xFF), (45, 40, 100, 8, 127, 0xFF),
                                 (33, 21, 130, 3, 0, 0x3C), (70, 25, 256, 20, 64, 0xFF),
-                                (17, 9, 5, 1, 2, 0x81), (90, 30, 190, 8, 32, 0xFF)]:
+                                (17, 9, 5, 1, 2, 0x81), (90, 30, 190, 8, 32, 0xFF),
+                                (48, 27, 64, 5, 40, 0x66), (40, 19, 32, 2, 90, 0x7F)]:
     C = rng.integers(0, 1 << 21, (H, W, D)).astype(np.uint32)
     C[rng.random(C.shape) < 0.5] = rng.integers(0, 300)
     x = np.arange(W)[None, :, None]

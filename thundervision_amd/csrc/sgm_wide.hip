@@ -494,8 +494,10 @@ __global__ __launch_bounds__(256) void sgm_wide_sum_wta_kernel(const uint8_t *__
 }
 
 // The same with 16 cells per thread (D % 16 == 0, 16-byte aligned cells):
-// 16-byte loads of C and of every plane.
-template <int P>
+// 16-byte loads of C and of every plane. N = 8: the eight planes' loads and
+// C's are all issued before any sum (the plane count a compile-time
+// constant); N = 0: a loop over the n planes of a direction mask.
+template <int P, int N>
 __global__ __launch_bounds__(256) void sgm_wide_sum_wta16_kernel(const uint8_t *__restrict__ C, int64_t csx,
                                                                  int64_t csy, const uint8_t *__restrict__ planes,
                                                                  size_t plane_bytes, int n, int W, int H, int D,
@@ -515,17 +517,36 @@ __global__ __launch_bounds__(256) void sgm_wide_sum_wta16_kernel(const uint8_t *
         uint32_t e[16];
 #pragma unroll
         for (int q = 0; q < 16; q++) e[q] = 0;
-        for (int r = 0; r < n; r++) {
-            const u32x4 w = __builtin_nontemporal_load((const u32x4 *)(ep + (size_t)r * plane_bytes));
+        u32x4 c4[4];
+        if constexpr (N > 0) {
+            u32x4 w[N];
 #pragma unroll
-            for (int q = 0; q < 16; q++) e[q] += (w[q / 4] >> (8 * (q % 4))) & 0xFFu;
+            for (int r = 0; r < N; r++)
+                w[r] = __builtin_nontemporal_load((const u32x4 *)(ep + (size_t)r * plane_bytes));
+#pragma unroll
+            for (int g = 0; g < 4; g++)
+                c4[g] = __builtin_nontemporal_load((const u32x4 *)(cp + 16 * g));
+#pragma unroll
+            for (int r = 0; r < N; r++)
+#pragma unroll
+                for (int q = 0; q < 16; q++) e[q] += (w[r][q / 4] >> (8 * (q % 4))) & 0xFFu;
+        } else {
+            for (int r = 0; r < n; r++) {
+                const u32x4 w = __builtin_nontemporal_load((const u32x4 *)(ep + (size_t)r * plane_bytes));
+#pragma unroll
+                for (int q = 0; q < 16; q++) e[q] += (w[q / 4] >> (8 * (q % 4))) & 0xFFu;
+            }
+#pragma unroll
+            for (int g = 0; g < 4; g++)
+                c4[g] = __builtin_nontemporal_load((const u32x4 *)(cp + 16 * g));
         }
+        const int nn = N > 0 ? N : n;
 #pragma unroll
         for (int g = 0; g < 4; g++) {
-            const u32x4 c = __builtin_nontemporal_load((const u32x4 *)(cp + 16 * g));
+            const u32x4 c = c4[g];
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const uint32_t k = wide_key(wide_s(c[q], e[4 * g + q], n));
+                const uint32_t k = wide_key(wide_s(c[q], e[4 * g + q], nn));
                 if (k < bv) {
                     bv = k;
                     bd = (uint32_t)(d + 4 * g + q);
@@ -670,7 +691,10 @@ static int run_wide_e(tdvh_ctx *ctx, const tdvh_dsi *C, const tdvh_sgm_params *p
             prof_begin(s);
             switch (P16) {
 #define CASE(q)                                                                                    \
-    case q: sgm_wide_sum_wta16_kernel<q><<<b16, 256, 0, s>>>(a.C, a.csx, a.csy, a.planes, a.plane_bytes, a.ndirs, W, H, D, disp, disp_pitch); break;
+    case q:                                                                                        \
+        if (a.ndirs == 8) sgm_wide_sum_wta16_kernel<q, 8><<<b16, 256, 0, s>>>(a.C, a.csx, a.csy, a.planes, a.plane_bytes, a.ndirs, W, H, D, disp, disp_pitch); \
+        else sgm_wide_sum_wta16_kernel<q, 0><<<b16, 256, 0, s>>>(a.C, a.csx, a.csy, a.planes, a.plane_bytes, a.ndirs, W, H, D, disp, disp_pitch); \
+        break;
                 CASE(1) CASE(2) CASE(4) CASE(8) CASE(16)
 #undef CASE
             }
