@@ -85,3 +85,36 @@ def test_pipeline_host_and_device_buffers_interleaved(tv, dev):
             assert (got == want(L, R, D, 3)).all(), k
     finally:
         p.close()
+
+
+def test_two_pipelines_on_two_streams(tv, dev):
+    """Two pipelines, each with its own context, stream, device buffers and
+    pinned output, take alternate frames with no synchronisation between
+    frames (so one stream's copies and kernels overlap the other's): every
+    frame equals the oracle chain."""
+    W, H, D = 160, 96, 48
+    frames = [synth.pair(W, H, D - 1, 5, 90 + k)[:2] for k in range(6)]
+    lanes = []
+    for _ in range(2):
+        ctx = tv.Context(0)
+        lanes.append(dict(ctx=ctx, st=torch.cuda.Stream(), p=tv.Pipeline(W, H, D, flags=3, ctx=ctx),
+                          L=torch.empty((H, W), dtype=torch.uint8, device=dev),
+                          R=torch.empty((H, W), dtype=torch.uint8, device=dev),
+                          out=torch.empty((H, W), dtype=torch.int16, device=dev)))
+    hosts = [(torch.from_numpy(L).pin_memory(), torch.from_numpy(R).pin_memory()) for L, R in frames]
+    outs = [torch.full((H, W), -1, dtype=torch.int16).pin_memory() for _ in frames]
+    try:
+        for k, ((Lh, Rh), oh) in enumerate(zip(hosts, outs)):
+            ln = lanes[k % 2]
+            with torch.cuda.stream(ln["st"]):
+                ln["L"].copy_(Lh, non_blocking=True)
+                ln["R"].copy_(Rh, non_blocking=True)
+                ln["p"].run(ln["L"], ln["R"], out=ln["out"], stream=ln["st"])
+                oh.copy_(ln["out"], non_blocking=True)
+        torch.cuda.synchronize()
+        for k, ((L, R), oh) in enumerate(zip(frames, outs)):
+            assert (oh.numpy() == want(L, R, D, 3)).all(), k
+    finally:
+        for ln in lanes:
+            ln["p"].close()
+            ln["ctx"].close()
